@@ -1,0 +1,192 @@
+/*
+ * parhip.h -- C ABI of the MI355X (gfx950) recovery-slice engine.
+ *
+ * Drop-in boundary for ParParPar's GF(2^64) PAR3 engine.  Each entry point
+ * replaces one member of `class GF64Controller` (src/par3_engine.h:11-244) or
+ * one region operation of gf64/gf64_global.h:23-29, with the same argument
+ * meaning, word layout and value semantics, so the reference's N-API addon
+ * (src/gf64_addon.cc) can bind it by swapping the call at each site.
+ *
+ * Differences from the reference, all additive:
+ *   - every entry returns an int status (PARHIP_OK = 0) where the reference
+ *     returns void and silently skips (src/par3_engine.cc:1069, :1173-1176);
+ *     the no-op cases (numInputs == 0, numRecovery == 0) still return
+ *     PARHIP_OK and touch nothing, as the reference does;
+ *   - `numThreads` is accepted and ignored (work is spread over the GPU);
+ *   - calls are re-entrant: no unsynchronised global state (the reference's
+ *     coefficient LRU and dispatch pointers are not, src/par3_engine.cc:107-112);
+ *   - `parhip_d_*` variants take device pointers and a hipStream_t (as void*),
+ *     for data already resident in HBM; they enqueue work and return without
+ *     synchronising.
+ *
+ * Words are native little-endian uint64_t; a slice of blockSize64 words is
+ * blockSize64*8 bytes; inputs are contiguous numInputs x blockSize64 words and
+ * recovery blocks contiguous numRecovery x blockSize64 words (par3_engine.h:14-23).
+ */
+#ifndef PARHIP_H
+#define PARHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PARHIP_OK 0
+#define PARHIP_ERR_SINGULAR (-1) /* singular matrix: the reference's -1            */
+#define PARHIP_ERR_ARG (-2)      /* invalid argument (null pointer, bad size)      */
+#define PARHIP_ERR_NOMEM (-3)    /* device or host allocation failed               */
+#define PARHIP_ERR_DEVICE (-4)   /* HIP runtime error                              */
+#define PARHIP_ERR_RANGE (-5)    /* size beyond a documented limit                 */
+
+/* ---- multiply strategies for the contraction ---------------------------- */
+#define PARHIP_METHOD_AUTO 0     /* per-shape choice (autotuner table)             */
+#define PARHIP_METHOD_TABLE 1    /* split nibble tables in LDS                     */
+#define PARHIP_METHOD_MFMA 2     /* GF(2) bit-matrix on MFMA                       */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int parhip_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char *parhip_last_error(void);
+/* Select the HIP device used by calls from this thread (default 0). */
+int parhip_set_device(int device);
+/* Force a contraction strategy for this thread (PARHIP_METHOD_*). */
+int parhip_set_method(int method);
+/* Release cached device memory (coefficient cache, workspaces). */
+int parhip_release_cache(void);
+
+/* ======================================================================= */
+/* GF64Controller, host pointers (synchronous: return after results land)   */
+/* ======================================================================= */
+
+/* GF64Controller::ComputeRecoveryBlocks (src/par3_engine.h:24-32,
+ * src/par3_engine.cc:1061-1111). accumulate=0 overwrites the recovery blocks,
+ * accumulate!=0 XORs into them (chunked create, lib/par3gen.js:2306-2316). */
+int parhip_gf64_compute_recovery_blocks(const uint64_t *inputs, size_t numInputs,
+                                        uint64_t *recovery, size_t numRecovery,
+                                        size_t blockSize64, uint64_t firstInput,
+                                        uint64_t firstRecovery, int numThreads, int accumulate);
+
+/* GF64Controller::ComputeRecoveryBlocksFull (src/par3_engine.h:49-57, :1389-1399). */
+int parhip_gf64_compute_recovery_blocks_full(const uint64_t *inputs, size_t numInputs,
+                                             uint64_t *recovery, size_t numRecovery,
+                                             size_t blockSize64, uint64_t firstInput,
+                                             uint64_t firstRecovery, int numThreads,
+                                             int accumulate);
+
+/* GF64Controller::ComputeRecoveryBlocksWithCoeff (src/par3_engine.h:65-72,
+ * src/par3_engine.cc:1165-1382); coeff is numRecovery x numInputs row-major. */
+int parhip_gf64_compute_recovery_blocks_with_coeff(const uint64_t *inputs, size_t numInputs,
+                                                   uint64_t *recovery, size_t numRecovery,
+                                                   size_t blockSize64, const uint64_t *coeff,
+                                                   int numThreads, int accumulate);
+
+/* GF64Controller::BuildCauchyMatrix (src/par3_engine.h:119-123, :358-429):
+ * M[r][c] = inv((firstInput+c) XOR (firstRecovery+r)), zero denominators -> 1. */
+int parhip_gf64_build_cauchy_matrix(uint64_t *coeffMatrix, size_t numInputs, size_t numRecovery,
+                                    uint64_t firstInput, uint64_t firstRecovery);
+
+/* GF64Controller::BuildCauchyMatrixAlloc (src/par3_engine.h:84-87, :432-441):
+ * malloc'd result (free with free() or parhip_free), NULL on N==0, R==0 or OOM. */
+uint64_t *parhip_gf64_build_cauchy_matrix_alloc(size_t numInputs, size_t numRecovery,
+                                                uint64_t firstInput, uint64_t firstRecovery);
+void parhip_free(void *p);
+
+/* GF64Controller::MultiplyAccumulate (src/par3_engine.h:133-138):
+ * out[i] ^= XOR_j in[j] * coeffMatrix[i*numIn + j]. */
+int parhip_gf64_multiply_accumulate(uint64_t *out, size_t numOut, const uint64_t *in,
+                                    size_t numIn, const uint64_t *coeffMatrix, size_t blockSize64);
+
+/* GF64Controller::SolveAndReconstruct (src/par3_engine.h:162-168, :980-1048):
+ * A (n x n, row-major) is eliminated in place exactly as the reference does it
+ * (first nonzero pivot, row swaps); rhs (n x blockSize64) becomes A^-1 * rhs.
+ * Returns PARHIP_OK (0), or PARHIP_ERR_SINGULAR (-1, the reference's value)
+ * with A and rhs left in the reference's partially-eliminated state. */
+int parhip_gf64_solve_and_reconstruct(uint64_t *A, uint64_t *rhsBlocks, size_t n,
+                                      size_t blockSize64, int numThreads);
+
+/* gf64_solve (gf64/gf64_solve.c:39-96): x = A^-1 b for one word per row. */
+int parhip_gf64_solve(const uint64_t *A, const uint64_t *b, uint64_t *x, size_t n);
+
+/* gf64_mul / gf64_inverse element operations (gf64/gf64_single.c:6-53,
+ * gf64/gf64_invert.c:29-67). Host computation. */
+uint64_t parhip_gf64_mul(uint64_t a, uint64_t b);
+uint64_t parhip_gf64_inverse(uint64_t a);
+
+/* ======================================================================= */
+/* Region operations (gf64/gf64_global.h:23-29), host pointers              */
+/* ======================================================================= */
+/* out[w] = in[w] * c */
+int parhip_gf64_region_mul(uint64_t *out, const uint64_t *in, size_t len, uint64_t constant);
+/* out[w] = in[w] * coeff[w % n_coeff]   (cycling, gf64/gf64_region_scalar.c:14-18) */
+int parhip_gf64_region_mul_arr(uint64_t *out, const uint64_t *in, const uint64_t *coeff,
+                               size_t len, size_t n_coeff);
+/* out[w] ^= XOR_c in[w] * coeff[c]     (sum, gf64/gf64_region_scalar.c:20-28) */
+int parhip_gf64_region_muladd_arr(uint64_t *out, const uint64_t *in, const uint64_t *coeff,
+                                  size_t len, size_t n_coeff);
+/* out[w] ^= XOR_g in_blocks[g][w] * coeff_blocks[g]   (gf64_region_scalar.c:45-55) */
+int parhip_gf64_region_coupled_muladd_arr(uint64_t *out, const uint64_t *const *in_blocks,
+                                          const uint64_t *coeff_blocks, size_t len, size_t G);
+/* outs[k][w] ^= in[w] * (*coeff_block_starts[k])      (gf64_region_scalar.c:58-72) */
+int parhip_gf64_region_fused_output_muladd_arr(uint64_t *const *outs, const uint64_t *in,
+                                               const uint64_t *const *coeff_block_starts,
+                                               size_t len, size_t K);
+/* outs[k][w] ^= XOR_g in_blocks[g][w] * coeff2d[k*K_stride + g]  (gf64_region_scalar.c:75-91) */
+int parhip_gf64_region_2d_muladd_arr(uint64_t *const *outs, size_t K,
+                                     const uint64_t *const *in_blocks, size_t G,
+                                     const uint64_t *coeff_block_2d, size_t K_stride, size_t len);
+
+/* ======================================================================= */
+/* Device-resident variants (pointers into HBM, stream = hipStream_t or NULL)*/
+/* ======================================================================= */
+
+/* Cauchy matrix into device memory d_coeff (numRecovery x numInputs). */
+int parhip_d_gf64_build_cauchy(uint64_t *d_coeff, size_t numInputs, size_t numRecovery,
+                               uint64_t firstInput, uint64_t firstRecovery, void *stream);
+
+/* Contraction with an explicit device coefficient matrix (ld = numInputs).
+ * Input i starts at d_inputs + i*inputStride64, output r at
+ * d_recovery + r*recoveryStride64 (strides in words, >= blockSize64). */
+int parhip_d_gf64_recovery_with_coeff(const uint64_t *d_inputs, size_t numInputs,
+                                      size_t inputStride64, uint64_t *d_recovery,
+                                      size_t numRecovery, size_t recoveryStride64,
+                                      size_t blockSize64, const uint64_t *d_coeff,
+                                      int accumulate, void *stream);
+
+/* Contraction with Cauchy coefficients for points (firstInput+i, firstRecovery+r);
+ * the coefficient matrix is built on the device and cached per shape. */
+int parhip_d_gf64_recovery(const uint64_t *d_inputs, size_t numInputs, size_t inputStride64,
+                           uint64_t *d_recovery, size_t numRecovery, size_t recoveryStride64,
+                           size_t blockSize64, uint64_t firstInput, uint64_t firstRecovery,
+                           int accumulate, void *stream);
+
+/* Contraction over scattered slices: d_in_ptrs / d_out_ptrs are device arrays
+ * of device pointers (e.g. the known blocks of an archive during repair);
+ * d_coeff is numOut x numIn with leading dimension coeff_ld. */
+int parhip_d_gf64_recovery_ptrs(const uint64_t *const *d_in_ptrs, size_t numIn,
+                                uint64_t *const *d_out_ptrs, size_t numOut, size_t blockSize64,
+                                const uint64_t *d_coeff, size_t coeff_ld, int accumulate,
+                                void *stream);
+
+/* ======================================================================= */
+/* Kernel timing (no counterpart in the reference, which only has wall-     */
+/* clock hooks, src/gf64_addon.cc:1583-1615). Per calling thread.           */
+/* ======================================================================= */
+/* Bracket every kernel launched by this thread with HIP events (on its stream). */
+int parhip_prof_enable(int on);
+/* Drop accumulated timings. */
+int parhip_prof_reset(void);
+/* Total device time and launch count of a kernel: "mac_tab", "xor_reduce",
+ * "cauchy". Synchronises with the recorded events. */
+int parhip_prof_get(const char *kernel, double *total_ms, uint64_t *launches);
+
+/* Device workspace the contraction of this shape needs (bytes). Informational. */
+size_t parhip_gf64_workspace_bytes(size_t numInputs, size_t numRecovery, size_t blockSize64);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARHIP_H */

@@ -1,0 +1,117 @@
+"""parparpar_amd -- MI355X (gfx950) recovery-slice engine for PAR3 / PAR2.
+
+The engine is the in-tree shared library ``parparpar_amd/libparhip.so`` (built
+by ``python -m parparpar_amd.build``) behind the C ABI in ``include/parhip.h``.
+This package only loads it and gives Python the same call shapes the
+reference's N-API addon exposes (``parparpar_amd.binding``); there is no
+Python or CPU fallback for any computation: if the library is missing every
+call raises.
+"""
+import ctypes
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libparhip.so")
+
+_lib = None
+
+_u64 = ctypes.c_uint64
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u64pp = ctypes.POINTER(_u64p)
+_sz = ctypes.c_size_t
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+
+# (name, restype, argtypes) of every C-ABI entry point the package binds.
+_SIGS = [
+    ("parhip_version", _int, []),
+    ("parhip_last_error", ctypes.c_char_p, []),
+    ("parhip_set_device", _int, [_int]),
+    ("parhip_set_method", _int, [_int]),
+    ("parhip_release_cache", _int, []),
+    ("parhip_free", None, [_vp]),
+    ("parhip_gf64_mul", _u64, [_u64, _u64]),
+    ("parhip_gf64_inverse", _u64, [_u64]),
+    ("parhip_gf64_compute_recovery_blocks", _int, [_vp, _sz, _vp, _sz, _sz, _u64, _u64, _int, _int]),
+    ("parhip_gf64_compute_recovery_blocks_full", _int, [_vp, _sz, _vp, _sz, _sz, _u64, _u64, _int, _int]),
+    ("parhip_gf64_compute_recovery_blocks_with_coeff", _int, [_vp, _sz, _vp, _sz, _sz, _vp, _int, _int]),
+    ("parhip_gf64_build_cauchy_matrix", _int, [_vp, _sz, _sz, _u64, _u64]),
+    ("parhip_gf64_build_cauchy_matrix_alloc", _vp, [_sz, _sz, _u64, _u64]),
+    ("parhip_gf64_multiply_accumulate", _int, [_vp, _sz, _vp, _sz, _vp, _sz]),
+    ("parhip_gf64_solve_and_reconstruct", _int, [_vp, _vp, _sz, _sz, _int]),
+    ("parhip_gf64_solve", _int, [_vp, _vp, _vp, _sz]),
+    ("parhip_gf64_region_mul", _int, [_vp, _vp, _sz, _u64]),
+    ("parhip_gf64_region_mul_arr", _int, [_vp, _vp, _vp, _sz, _sz]),
+    ("parhip_gf64_region_muladd_arr", _int, [_vp, _vp, _vp, _sz, _sz]),
+    ("parhip_gf64_region_coupled_muladd_arr", _int, [_vp, _vp, _vp, _sz, _sz]),
+    ("parhip_gf64_region_fused_output_muladd_arr", _int, [_vp, _vp, _vp, _sz, _sz]),
+    ("parhip_gf64_region_2d_muladd_arr", _int, [_vp, _sz, _vp, _sz, _vp, _sz, _sz]),
+    ("parhip_d_gf64_build_cauchy", _int, [_vp, _sz, _sz, _u64, _u64, _vp]),
+    ("parhip_d_gf64_recovery_with_coeff", _int, [_vp, _sz, _sz, _vp, _sz, _sz, _sz, _vp, _int, _vp]),
+    ("parhip_d_gf64_recovery", _int, [_vp, _sz, _sz, _vp, _sz, _sz, _sz, _u64, _u64, _int, _vp]),
+    ("parhip_d_gf64_recovery_ptrs", _int, [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _int, _vp]),
+    ("parhip_gf64_workspace_bytes", _sz, [_sz, _sz, _sz]),
+    ("parhip_prof_enable", _int, [_int]),
+    ("parhip_prof_reset", _int, []),
+    ("parhip_prof_get", _int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
+]
+
+PARHIP_OK = 0
+PARHIP_ERR_SINGULAR = -1
+PARHIP_ERR_ARG = -2
+PARHIP_ERR_NOMEM = -3
+PARHIP_ERR_DEVICE = -4
+PARHIP_ERR_RANGE = -5
+
+METHOD_AUTO = 0
+METHOD_TABLE = 1
+METHOD_MFMA = 2
+
+
+class ParhipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("parhip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def exported_symbols():
+    return [name for name, _, _ in _SIGS]
+
+
+def lib():
+    """The loaded engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("parparpar_amd/libparhip.so is not built; run "
+                               "`python -m parparpar_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def prof_enable(on=True):
+    check(lib().parhip_prof_enable(1 if on else 0))
+
+
+def prof_reset():
+    check(lib().parhip_prof_reset())
+
+
+def prof_get(kernel):
+    """(total_ms, launches) of a named kernel on this thread since the last reset."""
+    ms = ctypes.c_double(0)
+    n = _u64(0)
+    check(lib().parhip_prof_get(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
+
+
+def check(rc):
+    if rc != PARHIP_OK:
+        msg = lib().parhip_last_error()
+        raise ParhipError(rc, msg.decode() if msg else "")
+    return rc

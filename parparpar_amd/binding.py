@@ -1,0 +1,174 @@
+"""The reference addon's call shapes over the gfx950 engine.
+
+ParParPar's JS front end calls ``parpar_gf64.node`` (src/gf64_addon.cc:3271-3612)
+with Buffers; this module offers the same functions with the same argument
+order over writable numpy buffers (uint8 or uint64), so tests can replay the
+reference's own call sequences.  Every call goes to libparhip.so.
+
+    compute_recovery(inputs, outputs, numInputs, numRecovery, blockSize,
+                     firstInput, firstRecovery[, numThreads])       gf64_addon.cc:1266
+    compute_recovery_full(...same...)                               gf64_addon.cc:1443
+    compute_recovery_accumulate(...same...)  (accumulate=true)      gf64_addon.cc:1799
+    compute_recovery_with_coeff(coeffMatrix, inputs, outputs, numInputs,
+                     numRecovery, blockSize, firstInput, firstRecovery[, numThreads])
+                                                                    gf64_addon.cc:2189
+    build_coefficient_matrix(numInputs, numRecovery, firstInput, firstRecovery)
+                                                                    gf64_addon.cc:2124
+    solve_and_reconstruct(A, rhsBlocks, n, blockSize) -> bool       gf64_addon.cc:2655
+    gf64_solve(A, b, n) -> x or None                                gf64_addon.cc:1127
+    mul_arr(out, inBuf, coeff, len_words, n_coeff)                  gf64_addon.cc:3038
+    Gf64Encoder().{mul, mul_arr, coupled_muladd_arr, fused_output_muladd_arr,
+                   two_d_muladd_arr}                                gf64_addon.cc:168-1054
+
+`blockSize` is in bytes and must be a multiple of 8, as the addon requires
+(gf64_addon.cc:1553-1578); violations raise ValueError (the addon's RangeError).
+"""
+import ctypes
+
+import numpy as np
+
+from . import check, lib
+
+
+def _buf(a, name):
+    if not isinstance(a, np.ndarray):
+        raise TypeError("%s must be a numpy buffer" % name)
+    if not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("%s must be contiguous" % name)
+    return a.ctypes.data
+
+
+def _check_sizes(buf, count, block, name):
+    if buf.nbytes < count * block:
+        raise ValueError("%s too small: %d < %d" % (name, buf.nbytes, count * block))
+
+
+def _check_block(blockSize):
+    if blockSize <= 0 or blockSize % 8:
+        raise ValueError("blockSize must be a positive multiple of 8")
+
+
+def compute_recovery(inputs, outputs, numInputs, numRecovery, blockSize, firstInput, firstRecovery,
+                     numThreads=0, accumulate=False):
+    _check_block(blockSize)
+    _check_sizes(inputs, numInputs, blockSize, "inputs")
+    _check_sizes(outputs, numRecovery, blockSize, "outputs")
+    if outputs.flags["WRITEABLE"] is False:
+        raise TypeError("outputs must be writable")
+    check(lib().parhip_gf64_compute_recovery_blocks(
+        _buf(inputs, "inputs"), numInputs, _buf(outputs, "outputs"), numRecovery, blockSize // 8,
+        firstInput, firstRecovery, numThreads, 1 if accumulate else 0))
+    return outputs
+
+
+def compute_recovery_full(inputs, outputs, numInputs, numRecovery, blockSize, firstInput,
+                          firstRecovery, numThreads=0, accumulate=False):
+    _check_block(blockSize)
+    _check_sizes(inputs, numInputs, blockSize, "inputs")
+    _check_sizes(outputs, numRecovery, blockSize, "outputs")
+    check(lib().parhip_gf64_compute_recovery_blocks_full(
+        _buf(inputs, "inputs"), numInputs, _buf(outputs, "outputs"), numRecovery, blockSize // 8,
+        firstInput, firstRecovery, numThreads, 1 if accumulate else 0))
+    return outputs
+
+
+def compute_recovery_accumulate(inputs, outputs, numInputs, numRecovery, blockSize, firstInput,
+                                firstRecovery, numThreads=0):
+    return compute_recovery_full(inputs, outputs, numInputs, numRecovery, blockSize, firstInput,
+                                 firstRecovery, numThreads, accumulate=True)
+
+
+def compute_recovery_with_coeff(coeffMatrix, inputs, outputs, numInputs, numRecovery, blockSize,
+                                firstInput=0, firstRecovery=0, numThreads=0, accumulate=False):
+    _check_block(blockSize)
+    _check_sizes(inputs, numInputs, blockSize, "inputs")
+    _check_sizes(outputs, numRecovery, blockSize, "outputs")
+    _check_sizes(coeffMatrix, numInputs * numRecovery, 8, "coeffMatrix")
+    check(lib().parhip_gf64_compute_recovery_blocks_with_coeff(
+        _buf(inputs, "inputs"), numInputs, _buf(outputs, "outputs"), numRecovery, blockSize // 8,
+        _buf(coeffMatrix, "coeffMatrix"), numThreads, 1 if accumulate else 0))
+    return outputs
+
+
+def build_coefficient_matrix(numInputs, numRecovery, firstInput, firstRecovery):
+    out = np.empty(numInputs * numRecovery, dtype=np.uint64)
+    check(lib().parhip_gf64_build_cauchy_matrix(out.ctypes.data, numInputs, numRecovery,
+                                                firstInput, firstRecovery))
+    return out
+
+
+def solve_and_reconstruct(A, rhsBlocks, n, blockSize):
+    """Returns True on success, False on a singular matrix (A, rhs modified in place)."""
+    if blockSize % 8:
+        raise ValueError("blockSize must be a multiple of 8")
+    _check_sizes(A, n * n, 8, "A")
+    _check_sizes(rhsBlocks, n, blockSize, "rhsBlocks")
+    rc = lib().parhip_gf64_solve_and_reconstruct(_buf(A, "A"), _buf(rhsBlocks, "rhsBlocks"), n,
+                                                 blockSize // 8, 0)
+    if rc == -1:
+        return False
+    check(rc)
+    return True
+
+
+def gf64_solve(A, b, n):
+    x = np.zeros(n, dtype=np.uint64)
+    A = np.ascontiguousarray(A, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    rc = lib().parhip_gf64_solve(A.ctypes.data, b.ctypes.data, x.ctypes.data, n)
+    if rc == -1:
+        return None
+    check(rc)
+    return x
+
+
+def mul_arr(out, inBuf, coeff, len_words, n_coeff):
+    check(lib().parhip_gf64_region_mul_arr(_buf(out, "out"), _buf(inBuf, "in"), _buf(coeff, "coeff"),
+                                           len_words, n_coeff))
+    return out
+
+
+def _ptr_array(bufs):
+    arr = (ctypes.c_void_p * len(bufs))(*[b.ctypes.data for b in bufs])
+    return arr
+
+
+class Gf64Encoder:
+    """Region operations (gf64/gf64_global.h:23-29) as the addon's Gf64Encoder exposes them."""
+
+    def mul(self, out, inBuf, constant, len_words):
+        check(lib().parhip_gf64_region_mul(_buf(out, "out"), _buf(inBuf, "in"), len_words, constant))
+        return out
+
+    def mul_arr(self, out, inBuf, coeff, len_words, n_coeff):
+        return mul_arr(out, inBuf, coeff, len_words, n_coeff)
+
+    def muladd_arr(self, out, inBuf, coeff, len_words, n_coeff):
+        check(lib().parhip_gf64_region_muladd_arr(_buf(out, "out"), _buf(inBuf, "in"),
+                                                  _buf(coeff, "coeff"), len_words, n_coeff))
+        return out
+
+    def coupled_muladd_arr(self, out, in_blocks, coeff_blocks, len_words):
+        ptrs = _ptr_array(in_blocks)
+        check(lib().parhip_gf64_region_coupled_muladd_arr(
+            _buf(out, "out"), ctypes.addressof(ptrs), _buf(coeff_blocks, "coeff"), len_words,
+            len(in_blocks)))
+        return out
+
+    def fused_output_muladd_arr(self, outs, inBuf, coeffs, len_words):
+        """outs[k] ^= in * coeffs[k] (coeffs: uint64 array; one pointer per output)."""
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        optrs = _ptr_array(outs)
+        cptrs = (ctypes.c_void_p * len(outs))(*[coeffs.ctypes.data + 8 * k for k in range(len(outs))])
+        check(lib().parhip_gf64_region_fused_output_muladd_arr(
+            ctypes.addressof(optrs), _buf(inBuf, "in"), ctypes.addressof(cptrs), len_words, len(outs)))
+        return outs
+
+    def two_d_muladd_arr(self, outs, in_blocks, coeff2d, K_stride, len_words):
+        coeff2d = np.ascontiguousarray(coeff2d, dtype=np.uint64)
+        optrs = _ptr_array(outs)
+        iptrs = _ptr_array(in_blocks)
+        check(lib().parhip_gf64_region_2d_muladd_arr(
+            ctypes.addressof(optrs), len(outs), ctypes.addressof(iptrs), len(in_blocks),
+            coeff2d.ctypes.data, K_stride, len_words))
+        return outs

@@ -1,0 +1,463 @@
+// engine.cpp -- C-ABI implementation of the GF(2^64) engine (include/parhip.h).
+//
+// Host side of the drop-in for GF64Controller (src/par3_engine.h): argument
+// validation, device memory, the per-shape launch plan, the coefficient cache,
+// and the host <-> HBM staging for the host-pointer entries.  All arithmetic
+// on slices runs in the HIP kernels (gf64_kernels.hip); this file only does
+// k x k pivoting on the host (SolveAndReconstruct, gf64_solve).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <list>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/parhip.h"
+#include "engine_internal.h"
+#include "gf64_host.h"
+#include "kernels.h"
+
+namespace parhip {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string t_err;
+static thread_local int t_device = 0;
+static thread_local int t_method = PARHIP_METHOD_AUTO;
+
+int fail(int code, const char* fmt, ...) {
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof buf, fmt, ap);
+	va_end(ap);
+	t_err = buf;
+	return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+	return fail(e == hipErrorOutOfMemory ? PARHIP_ERR_NOMEM : PARHIP_ERR_DEVICE, "%s: %s", what,
+	            hipGetErrorString(e));
+}
+
+int current_device() { return t_device; }
+int current_method() { return t_method; }
+
+// ------------------------------------------------------------------ device info
+struct DevInfo {
+	int cus = 256;
+	bool ready = false;
+};
+static std::mutex g_dev_mu;
+static std::map<int, DevInfo> g_dev;
+
+int device_cus(int dev) {
+	std::lock_guard<std::mutex> lk(g_dev_mu);
+	DevInfo& d = g_dev[dev];
+	if (!d.ready) {
+		hipDeviceProp_t p;
+		if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0)
+			d.cus = p.multiProcessorCount;
+		// keep freed stream-ordered allocations cached in the pool
+		hipMemPool_t pool;
+		if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+			uint64_t thr = UINT64_MAX;
+			hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+		}
+		d.ready = true;
+	}
+	return d.cus;
+}
+
+// Per-thread stream for the synchronous host-pointer entries.
+struct ThreadStream {
+	hipStream_t s = nullptr;
+	int dev = -1;
+	~ThreadStream() {
+		if (s) hipStreamDestroy(s);
+	}
+};
+static thread_local ThreadStream t_stream;
+
+int host_stream(hipStream_t* out) {
+	int dev = t_device;
+	hipError_t e = hipSetDevice(dev);
+	if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+	if (t_stream.s == nullptr || t_stream.dev != dev) {
+		if (t_stream.s) hipStreamDestroy(t_stream.s);
+		t_stream.s = nullptr;
+		e = hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking);
+		if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+		t_stream.dev = dev;
+	}
+	device_cus(dev);
+	*out = t_stream.s;
+	return PARHIP_OK;
+}
+
+// ------------------------------------------------------------------ memory
+int dev_alloc(void** p, size_t bytes, hipStream_t s) {
+	*p = nullptr;
+	if (bytes == 0) return PARHIP_OK;
+	hipError_t e = hipMallocAsync(p, bytes, s);
+	if (e != hipSuccess) {
+		*p = nullptr;
+		return hip_fail(e, "hipMallocAsync");
+	}
+	return PARHIP_OK;
+}
+
+void dev_free(void* p, hipStream_t s) {
+	if (p) hipFreeAsync(p, s);
+}
+
+// ------------------------------------------------------------------ coefficient cache
+// Device-resident Cauchy matrices keyed by (device, N, R, firstInput,
+// firstRecovery), LRU of 16 entries.  Mirrors the reference's s_coeffCache
+// (src/par3_engine.cc:451-482) but guarded by a mutex, so concurrent callers
+// are safe.  Entries are built on the device (k_cauchy); a user waits on the
+// entry's `built` event, and an entry is only evicted when no caller holds it
+// (reference count) and after the device has drained.
+struct CoeffKey {
+	int dev;
+	uint64_t n, r, fi, fr;
+	bool operator<(const CoeffKey& o) const {
+		return std::tie(dev, n, r, fi, fr) < std::tie(o.dev, o.n, o.r, o.fi, o.fr);
+	}
+	bool operator==(const CoeffKey& o) const {
+		return std::tie(dev, n, r, fi, fr) == std::tie(o.dev, o.n, o.r, o.fi, o.fr);
+	}
+};
+struct CoeffEntry {
+	uint64_t* d = nullptr;
+	int refs = 0;
+	hipEvent_t built = nullptr;  // recorded once, after k_cauchy
+};
+static std::mutex g_coeff_mu;
+static std::map<CoeffKey, CoeffEntry> g_coeff;
+static std::list<CoeffKey> g_coeff_lru;
+static const size_t kCoeffCacheMax = 16;
+
+int coeff_acquire(uint64_t n, uint64_t r, uint64_t fi, uint64_t fr, hipStream_t s,
+                  const uint64_t** out) {
+	CoeffKey key{t_device, n, r, fi, fr};
+	std::unique_lock<std::mutex> lk(g_coeff_mu);
+	auto it = g_coeff.find(key);
+	if (it != g_coeff.end()) {
+		it->second.refs++;
+		g_coeff_lru.remove(key);
+		g_coeff_lru.push_front(key);
+		hipStreamWaitEvent(s, it->second.built, 0);
+		*out = it->second.d;
+		return PARHIP_OK;
+	}
+	// evict unreferenced entries beyond the cap
+	while (g_coeff.size() >= kCoeffCacheMax) {
+		bool evicted = false;
+		for (auto lit = g_coeff_lru.rbegin(); lit != g_coeff_lru.rend(); ++lit) {
+			auto eit = g_coeff.find(*lit);
+			if (eit != g_coeff.end() && eit->second.refs == 0) {
+				hipDeviceSynchronize();
+				hipFree(eit->second.d);
+				hipEventDestroy(eit->second.built);
+				g_coeff.erase(eit);
+				g_coeff_lru.erase(std::next(lit).base());
+				evicted = true;
+				break;
+			}
+		}
+		if (!evicted) break;
+	}
+	CoeffEntry ent;
+	hipError_t e = hipMalloc((void**)&ent.d, n * r * sizeof(uint64_t));
+	if (e != hipSuccess) return hip_fail(e, "hipMalloc(coefficients)");
+	{
+		ProfScope ps("cauchy", s);
+		e = launch_cauchy(ent.d, n, r, fi, fr, s);
+	}
+	if (e != hipSuccess) {
+		hipFree(ent.d);
+		return hip_fail(e, "k_cauchy");
+	}
+	hipEventCreateWithFlags(&ent.built, hipEventDisableTiming);
+	hipEventRecord(ent.built, s);
+	ent.refs = 1;
+	g_coeff[key] = ent;
+	g_coeff_lru.push_front(key);
+	*out = ent.d;
+	return PARHIP_OK;
+}
+
+void coeff_release(const uint64_t* d) {
+	std::lock_guard<std::mutex> lk(g_coeff_mu);
+	for (auto& kv : g_coeff)
+		if (kv.second.d == d) {
+			kv.second.refs--;
+			return;
+		}
+}
+
+void coeff_cache_clear() {
+	std::lock_guard<std::mutex> lk(g_coeff_mu);
+	for (auto it = g_coeff.begin(); it != g_coeff.end();) {
+		if (it->second.refs == 0) {
+			hipDeviceSynchronize();
+			hipFree(it->second.d);
+			hipEventDestroy(it->second.built);
+			g_coeff_lru.remove(it->first);
+			it = g_coeff.erase(it);
+		} else {
+			++it;
+		}
+	}
+}
+
+// ------------------------------------------------------------------ kernel timing
+// Per-thread HIP-event timing of named kernels, enabled by parhip_prof_enable.
+// Events are recorded on the stream the kernel is launched on, so the figure
+// is the kernel's own device duration (plus launch gaps inside the bracket).
+struct ProfRec {
+	std::string name;
+	hipEvent_t a, b;
+};
+static thread_local bool t_prof = false;
+static thread_local std::vector<ProfRec> t_prof_pending;
+static thread_local std::map<std::string, std::pair<double, uint64_t>> t_prof_tot;
+
+ProfScope::ProfScope(const char* name, hipStream_t s) : s_(s) {
+	if (!t_prof) return;
+	name_ = name;
+	hipEventCreate(&a_);
+	hipEventCreate(&b_);
+	hipEventRecord(a_, s_);
+	on_ = true;
+}
+ProfScope::~ProfScope() {
+	if (!on_) return;
+	hipEventRecord(b_, s_);
+	t_prof_pending.push_back({name_, a_, b_});
+}
+
+static void prof_collect() {
+	for (auto& r : t_prof_pending) {
+		hipEventSynchronize(r.b);
+		float ms = 0.f;
+		hipEventElapsedTime(&ms, r.a, r.b);
+		auto& t = t_prof_tot[r.name];
+		t.first += ms;
+		t.second += 1;
+		hipEventDestroy(r.a);
+		hipEventDestroy(r.b);
+	}
+	t_prof_pending.clear();
+}
+
+// ------------------------------------------------------------------ contraction plan
+Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev) {
+	Plan p;
+	p.method = PARHIP_METHOD_TABLE;
+	p.rt = R >= 16 ? 16 : (R > 4 ? 8 : (R > 2 ? 4 : 2));
+	p.wpt = W >= 1024 ? 4 : 2;
+	const uint64_t tile = 256ull * p.wpt;
+	const uint64_t tiles = (W + tile - 1) / tile;
+	const uint64_t rtiles = (R + p.rt - 1) / p.rt;
+	const int ib = p.rt <= 8 ? 4 : 2;
+	// work-groups resident per CU: LDS-bound (IB*RT*2 KiB per group, 160 KiB per CU)
+	const int lds_kib = ib * p.rt * 2;
+	int per_cu = 160 / lds_kib;
+	if (per_cu < 1) per_cu = 1;
+	if (per_cu > 4) per_cu = 4;
+	const uint64_t target = (uint64_t)device_cus(dev) * per_cu;
+	uint64_t groups = 1;
+	const uint64_t base = tiles * rtiles;
+	if (base < target) groups = (target + base - 1) / base;
+	// at least 2*IB inputs per group, so table building stays amortised
+	const uint64_t max_groups = std::max<uint64_t>(1, N / (2 * ib));
+	groups = std::min(groups, max_groups);
+	groups = std::min<uint64_t>(groups, 65535);
+	uint64_t per = (N + groups - 1) / groups;
+	per = (per + ib - 1) / ib * ib;
+	groups = (N + per - 1) / per;
+	p.groups = groups;
+	p.in_per_group = per;
+	return p;
+}
+
+size_t plan_workspace_bytes(const Plan& p, uint64_t R, uint64_t W) {
+	return p.groups > 1 ? (size_t)(p.groups * R * W * sizeof(uint64_t)) : 0;
+}
+
+// Enqueue the full contraction described by `a` (partial/in_per_group are
+// filled in here) on stream s.
+int run_contraction(MacArgs a, hipStream_t s) {
+	if (a.N == 0 || a.R == 0) return PARHIP_OK;
+	if (a.W == 0) return PARHIP_OK;
+	Plan p = plan_contraction(a.N, a.R, a.W, t_device);
+	a.in_per_group = p.in_per_group;
+	a.partial = nullptr;
+	void* ws = nullptr;
+	if (p.groups > 1) {
+		int rc = dev_alloc(&ws, plan_workspace_bytes(p, a.R, a.W), s);
+		if (rc) return rc;
+		a.partial = (uint64_t*)ws;
+	}
+	hipError_t e;
+	{
+		ProfScope ps("mac_tab", s);
+		e = launch_mac_tab(a, p.groups, p.rt, p.wpt, s);
+	}
+	if (e != hipSuccess) {
+		dev_free(ws, s);
+		return hip_fail(e, "k_mac_tab");
+	}
+	if (p.groups > 1) {
+		ProfScope ps("xor_reduce", s);
+		e = launch_xor_reduce(a.partial, p.groups, a.R, a.W, a.out_base, a.out_stride, a.out_ptrs,
+		                      a.accumulate, s);
+		dev_free(ws, s);
+		if (e != hipSuccess) return hip_fail(e, "k_xor_reduce");
+	}
+	return PARHIP_OK;
+}
+
+}  // namespace parhip
+
+using namespace parhip;
+
+// ====================================================================== C ABI
+extern "C" {
+
+int parhip_version(void) { return 100; }
+
+const char* parhip_last_error(void) { return t_err.c_str(); }
+
+int parhip_set_device(int device) {
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+		return fail(PARHIP_ERR_ARG, "parhip_set_device: no device %d", device);
+	t_device = device;
+	return PARHIP_OK;
+}
+
+int parhip_set_method(int method) {
+	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_MFMA)
+		return fail(PARHIP_ERR_ARG, "parhip_set_method: unknown method %d", method);
+	t_method = method;
+	return PARHIP_OK;
+}
+
+int parhip_prof_enable(int on) {
+	t_prof = on != 0;
+	return PARHIP_OK;
+}
+
+int parhip_prof_reset(void) {
+	prof_collect();
+	t_prof_tot.clear();
+	return PARHIP_OK;
+}
+
+int parhip_prof_get(const char* kernel, double* total_ms, uint64_t* launches) {
+	if (!kernel || !total_ms || !launches) return fail(PARHIP_ERR_ARG, "prof_get: null pointer");
+	prof_collect();
+	auto it = t_prof_tot.find(kernel);
+	*total_ms = it == t_prof_tot.end() ? 0.0 : it->second.first;
+	*launches = it == t_prof_tot.end() ? 0 : it->second.second;
+	return PARHIP_OK;
+}
+
+int parhip_release_cache(void) {
+	coeff_cache_clear();
+	return PARHIP_OK;
+}
+
+void parhip_free(void* p) { free(p); }
+
+uint64_t parhip_gf64_mul(uint64_t a, uint64_t b) { return host_gf64_mul(a, b); }
+uint64_t parhip_gf64_inverse(uint64_t a) { return host_gf64_inv(a); }
+
+size_t parhip_gf64_workspace_bytes(size_t numInputs, size_t numRecovery, size_t blockSize64) {
+	Plan p = plan_contraction(numInputs, numRecovery, blockSize64, t_device);
+	return plan_workspace_bytes(p, numRecovery, blockSize64);
+}
+
+// ------------------------------------------------------------- device entries
+int parhip_d_gf64_build_cauchy(uint64_t* d_coeff, size_t numInputs, size_t numRecovery,
+                               uint64_t firstInput, uint64_t firstRecovery, void* stream) {
+	if (numInputs == 0 || numRecovery == 0) return PARHIP_OK;
+	if (!d_coeff) return fail(PARHIP_ERR_ARG, "build_cauchy: null output");
+	ProfScope ps("cauchy", (hipStream_t)stream);
+	hipError_t e = launch_cauchy(d_coeff, numInputs, numRecovery, firstInput, firstRecovery,
+	                             (hipStream_t)stream);
+	return e == hipSuccess ? PARHIP_OK : hip_fail(e, "k_cauchy");
+}
+
+int parhip_d_gf64_recovery_with_coeff(const uint64_t* d_inputs, size_t numInputs,
+                                      size_t inputStride64, uint64_t* d_recovery,
+                                      size_t numRecovery, size_t recoveryStride64,
+                                      size_t blockSize64, const uint64_t* d_coeff, int accumulate,
+                                      void* stream) {
+	if (numInputs == 0 || numRecovery == 0) return PARHIP_OK;
+	if (!d_inputs || !d_recovery || !d_coeff)
+		return fail(PARHIP_ERR_ARG, "recovery_with_coeff: null pointer");
+	if (inputStride64 < blockSize64 || recoveryStride64 < blockSize64)
+		return fail(PARHIP_ERR_ARG, "recovery_with_coeff: stride < blockSize64");
+	device_cus(t_device);
+	MacArgs a{};
+	a.in_base = d_inputs;
+	a.in_stride = inputStride64;
+	a.out_base = d_recovery;
+	a.out_stride = recoveryStride64;
+	a.coeff = d_coeff;
+	a.coeff_ld = numInputs;
+	a.N = numInputs;
+	a.R = numRecovery;
+	a.W = blockSize64;
+	a.accumulate = accumulate ? 1 : 0;
+	return run_contraction(a, (hipStream_t)stream);
+}
+
+int parhip_d_gf64_recovery(const uint64_t* d_inputs, size_t numInputs, size_t inputStride64,
+                           uint64_t* d_recovery, size_t numRecovery, size_t recoveryStride64,
+                           size_t blockSize64, uint64_t firstInput, uint64_t firstRecovery,
+                           int accumulate, void* stream) {
+	if (numInputs == 0 || numRecovery == 0) return PARHIP_OK;
+	hipStream_t s = (hipStream_t)stream;
+	const uint64_t* coeff = nullptr;
+	int rc = coeff_acquire(numInputs, numRecovery, firstInput, firstRecovery, s, &coeff);
+	if (rc) return rc;
+	rc = parhip_d_gf64_recovery_with_coeff(d_inputs, numInputs, inputStride64, d_recovery,
+	                                       numRecovery, recoveryStride64, blockSize64, coeff,
+	                                       accumulate, stream);
+	coeff_release(coeff);
+	return rc;
+}
+
+int parhip_d_gf64_recovery_ptrs(const uint64_t* const* d_in_ptrs, size_t numIn,
+                                uint64_t* const* d_out_ptrs, size_t numOut, size_t blockSize64,
+                                const uint64_t* d_coeff, size_t coeff_ld, int accumulate,
+                                void* stream) {
+	if (numIn == 0 || numOut == 0) return PARHIP_OK;
+	if (!d_in_ptrs || !d_out_ptrs || !d_coeff)
+		return fail(PARHIP_ERR_ARG, "recovery_ptrs: null pointer");
+	if (coeff_ld < numIn) return fail(PARHIP_ERR_ARG, "recovery_ptrs: coeff_ld < numIn");
+	device_cus(t_device);
+	MacArgs a{};
+	a.in_ptrs = d_in_ptrs;
+	a.out_ptrs = d_out_ptrs;
+	a.coeff = d_coeff;
+	a.coeff_ld = coeff_ld;
+	a.N = numIn;
+	a.R = numOut;
+	a.W = blockSize64;
+	a.accumulate = accumulate ? 1 : 0;
+	return run_contraction(a, (hipStream_t)stream);
+}
+
+}  // extern "C"

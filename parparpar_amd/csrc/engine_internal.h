@@ -1,0 +1,65 @@
+// engine_internal.h -- shared host-side helpers of the C-ABI implementation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace parhip {
+
+struct Plan {
+	int method;            // PARHIP_METHOD_TABLE / PARHIP_METHOD_MFMA
+	int rt, wpt;           // outputs per work-group, words per thread
+	uint64_t groups;       // split-K work-group rows
+	uint64_t in_per_group; // inputs per row
+};
+
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+int current_device();
+int current_method();
+int device_cus(int dev);
+int host_stream(hipStream_t* out);
+int dev_alloc(void** p, size_t bytes, hipStream_t s);
+void dev_free(void* p, hipStream_t s);
+int coeff_acquire(uint64_t n, uint64_t r, uint64_t fi, uint64_t fr, hipStream_t s,
+                  const uint64_t** out);
+void coeff_release(const uint64_t* d);
+void coeff_cache_clear();
+Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev);
+size_t plan_workspace_bytes(const Plan& p, uint64_t R, uint64_t W);
+int run_contraction(MacArgs a, hipStream_t s);
+
+// Brackets a kernel launch with HIP events when timing is enabled
+// (parhip_prof_enable); no-op otherwise.
+class ProfScope {
+public:
+	ProfScope(const char* name, hipStream_t s);
+	~ProfScope();
+
+private:
+	hipStream_t s_;
+	const char* name_ = nullptr;
+	hipEvent_t a_ = nullptr, b_ = nullptr;
+	bool on_ = false;
+};
+
+// RAII device buffer on a stream (stream-ordered allocation).
+struct DevBuf {
+	void* p = nullptr;
+	hipStream_t s = nullptr;
+	DevBuf() = default;
+	DevBuf(const DevBuf&) = delete;
+	DevBuf& operator=(const DevBuf&) = delete;
+	int alloc(size_t bytes, hipStream_t st) {
+		s = st;
+		return dev_alloc(&p, bytes, st);
+	}
+	~DevBuf() { dev_free(p, s); }
+	template <class T>
+	T* as() const {
+		return (T*)p;
+	}
+};
+
+}  // namespace parhip

@@ -1,0 +1,291 @@
+// gf64_kernels.hip -- GF(2^64) recovery-slice kernels for gfx950 (MI355X).
+//
+// The hot operation is the contraction
+//     R[r][w] ^= XOR_i c(r,i) * D[i][w]          (64-bit LE words, GF(2^64))
+// behind ComputeRecoveryBlocks* / SolveAndReconstruct / the region ops of the
+// reference (src/par3_engine.cc:784-911, gf64/gf64_region_scalar.c:8-91).
+//
+// Strategy "nibble tables" (k_mac_tab): a work-group owns a tile of words and
+// a tile of RT outputs.  For every input it stages, per coefficient, sixteen
+// 16-entry split tables T_k[n] = c * (n x^{4k}) in LDS, two outputs interleaved
+// per 16-byte entry, so one ds_read_b128 serves two outputs and a 16-entry
+// table occupies exactly one 256-byte LDS row (conflict-free for any mix of
+// nibble values across the wave).  Each word of input is read from HBM once
+// (128-bit loads) and applied to all RT accumulators held in VGPRs.  Splitting
+// the input range over work-groups (split-K) gives enough work-groups for 256
+// CUs; partial results are XOR-reduced by k_xor_reduce.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "gf64_dev.h"
+#include "kernels.h"
+
+namespace parhip {
+
+// --------------------------------------------------------------------------
+// Cauchy coefficients (src/par3_engine.cc:358-429):
+//   M[r][c] = inv((firstInput + c) XOR (firstRecovery + r)), 0 -> 1.
+// --------------------------------------------------------------------------
+__global__ void k_cauchy(uint64_t* __restrict__ out, uint64_t N, uint64_t R, uint64_t fi,
+                         uint64_t fr) {
+	uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	uint64_t total = N * R;
+	for (; idx < total; idx += (uint64_t)gridDim.x * blockDim.x) {
+		uint64_t r = idx / N, c = idx - r * N;
+		uint64_t d = (fi + c) ^ (fr + r);
+		out[idx] = gf64_inv(d ? d : 1);
+	}
+}
+
+// Same coefficients for an arbitrary list of input points and recovery points
+// (repair: rows = available recovery ids, columns = missing or known inputs).
+__global__ void k_cauchy_points(uint64_t* __restrict__ out, const uint64_t* __restrict__ xs,
+                                uint64_t nx, const uint64_t* __restrict__ ys, uint64_t ny) {
+	uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	uint64_t total = nx * ny;
+	for (; idx < total; idx += (uint64_t)gridDim.x * blockDim.x) {
+		uint64_t r = idx / nx, c = idx - r * nx;
+		uint64_t d = xs[c] ^ ys[r];
+		out[idx] = gf64_inv(d ? d : 1);
+	}
+}
+
+// --------------------------------------------------------------------------
+// Nibble-table contraction.
+// --------------------------------------------------------------------------
+template <int RT, int WPT, int IB>
+__global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
+	static_assert(RT % 2 == 0 && RT >= 2 && RT <= 16, "RT");
+	static_assert(WPT % 2 == 0 && WPT >= 2 && WPT <= 8, "WPT");
+	constexpr int NP = RT / 2;               // output pairs
+	constexpr int TILE = 256 * WPT;          // words per work-group
+	// tab[ib][p][k][n] : 16 bytes = (c_{2p} * n x^{4k}, c_{2p+1} * n x^{4k})
+	__shared__ uint4 tab[IB][NP][16][16];
+
+	const int tid = threadIdx.x;
+	const uint64_t w0 = (uint64_t)blockIdx.x * TILE;
+	const uint64_t r0 = (uint64_t)blockIdx.z * RT;
+	const uint64_t g = blockIdx.y;
+	const uint64_t i_begin = g * a.in_per_group;
+	uint64_t i_end = i_begin + a.in_per_group;
+	if (i_end > a.N) i_end = a.N;
+
+	// word positions owned by this thread: pairs (2*tid + 512*j, +1)
+	uint64_t acc[WPT][RT];
+#pragma unroll
+	for (int j = 0; j < WPT; j++)
+#pragma unroll
+		for (int r = 0; r < RT; r++) acc[j][r] = 0;
+
+	for (uint64_t ib0 = i_begin; ib0 < i_end; ib0 += IB) {
+		// ---- issue data loads for this batch (in flight during the table build)
+		uint64_t d[IB][WPT];
+#pragma unroll
+		for (int ib = 0; ib < IB; ib++) {
+			uint64_t i = ib0 + ib;
+			const uint64_t* src = nullptr;
+			if (i < i_end) src = a.in_ptrs ? a.in_ptrs[i] : a.in_base + i * a.in_stride;
+#pragma unroll
+			for (int j = 0; j < WPT / 2; j++) {
+				uint64_t w = w0 + 2 * tid + 512 * j;
+				uint64_t v0 = 0, v1 = 0;
+				if (src) {
+					if (w + 1 < a.W && ((reinterpret_cast<uintptr_t>(src) & 15u) == 0)) {
+						const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(src + w);
+						v0 = v.x;
+						v1 = v.y;
+					} else {
+						if (w < a.W) v0 = src[w];
+						if (w + 1 < a.W) v1 = src[w + 1];
+					}
+				}
+				d[ib][2 * j] = v0;
+				d[ib][2 * j + 1] = v1;
+			}
+		}
+		// ---- build the split tables for IB inputs x RT outputs
+		__syncthreads();  // previous batch finished reading tab
+		for (int t = tid; t < IB * NP * 16; t += 256) {
+			const int k = t & 15;
+			const int p = (t >> 4) % NP;
+			const int ib = (t >> 4) / NP;
+			const uint64_t i = ib0 + ib;
+			uint64_t c0 = 0, c1 = 0;
+			if (i < i_end) {
+				const uint64_t ra = r0 + 2 * p, rb = ra + 1;
+				if (ra < a.R) c0 = a.coeff[ra * a.coeff_ld + i];
+				if (rb < a.R) c1 = a.coeff[rb * a.coeff_ld + i];
+			}
+			uint64_t b0[4], b1[4];
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				b0[q] = gf64_mulx_pow(c0, 4 * k + q);
+				b1[q] = gf64_mulx_pow(c1, 4 * k + q);
+			}
+			uint64_t e0[16], e1[16];
+			e0[0] = 0;
+			e1[0] = 0;
+#pragma unroll
+			for (int n = 1; n < 16; n++) {
+				const int low = __builtin_ctz(n);
+				e0[n] = e0[n & (n - 1)] ^ b0[low];
+				e1[n] = e1[n & (n - 1)] ^ b1[low];
+			}
+#pragma unroll
+			for (int n = 0; n < 16; n++) {
+				uint4 v;
+				v.x = (uint32_t)e0[n];
+				v.y = (uint32_t)(e0[n] >> 32);
+				v.z = (uint32_t)e1[n];
+				v.w = (uint32_t)(e1[n] >> 32);
+				tab[ib][p][k][n] = v;
+			}
+		}
+		__syncthreads();
+		// ---- lookups
+#pragma unroll
+		for (int ib = 0; ib < IB; ib++) {
+#pragma unroll
+			for (int j = 0; j < WPT; j++) {
+				const uint64_t x = d[ib][j];
+#pragma unroll
+				for (int k = 0; k < 16; k++) {
+					const unsigned n = (unsigned)(x >> (4 * k)) & 15u;
+#pragma unroll
+					for (int p = 0; p < NP; p++) {
+						const uint4 v = tab[ib][p][k][n];
+						acc[j][2 * p] ^= ((uint64_t)v.y << 32) | v.x;
+						acc[j][2 * p + 1] ^= ((uint64_t)v.w << 32) | v.z;
+					}
+				}
+			}
+		}
+	}
+
+	// ---- write out: partial buffer (split-K) or the outputs themselves
+#pragma unroll
+	for (int r = 0; r < RT; r++) {
+		const uint64_t rr = r0 + r;
+		if (rr >= a.R) break;
+		uint64_t* dst;
+		bool xr;
+		if (a.partial) {
+			dst = a.partial + (g * a.R + rr) * a.W;
+			xr = false;
+		} else {
+			dst = a.out_ptrs ? a.out_ptrs[rr] : a.out_base + rr * a.out_stride;
+			xr = a.accumulate != 0;
+		}
+#pragma unroll
+		for (int j = 0; j < WPT / 2; j++) {
+			const uint64_t w = w0 + 2 * tid + 512 * j;
+			if (w + 1 < a.W && ((reinterpret_cast<uintptr_t>(dst) & 15u) == 0)) {
+				ulonglong2* q = reinterpret_cast<ulonglong2*>(dst + w);
+				ulonglong2 v;
+				v.x = acc[2 * j][r];
+				v.y = acc[2 * j + 1][r];
+				if (xr) {
+					const ulonglong2 o = *q;
+					v.x ^= o.x;
+					v.y ^= o.y;
+				}
+				*q = v;
+			} else {
+				if (w < a.W) dst[w] = xr ? (dst[w] ^ acc[2 * j][r]) : acc[2 * j][r];
+				if (w + 1 < a.W) dst[w + 1] = xr ? (dst[w + 1] ^ acc[2 * j + 1][r]) : acc[2 * j + 1][r];
+			}
+		}
+	}
+}
+
+// out[r][w] = (accumulate ? out[r][w] : 0) ^ XOR_g partial[g][r][w]
+__global__ void k_xor_reduce(const uint64_t* __restrict__ partial, uint64_t G, uint64_t R,
+                             uint64_t W, uint64_t* out_base, uint64_t out_stride,
+                             uint64_t* const* out_ptrs, int accumulate) {
+	const uint64_t total = R * W;
+	for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+	     idx += (uint64_t)gridDim.x * blockDim.x) {
+		const uint64_t r = idx / W, w = idx - r * W;
+		uint64_t s = 0;
+		for (uint64_t g = 0; g < G; g++) s ^= partial[(g * R + r) * W + w];
+		uint64_t* dst = out_ptrs ? out_ptrs[r] : out_base + r * out_stride;
+		dst[w] = accumulate ? (dst[w] ^ s) : s;
+	}
+}
+
+// out[w] = in[w] * coeff[w % n]  (gf64/gf64_region_scalar.c:14-18, cycling coefficients)
+__global__ void k_region_mul_arr(uint64_t* __restrict__ out, const uint64_t* __restrict__ in,
+                                 const uint64_t* __restrict__ coeff, uint64_t len,
+                                 uint64_t n_coeff) {
+	for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < len;
+	     w += (uint64_t)gridDim.x * blockDim.x)
+		out[w] = gf64_mul(in[w], coeff[w % n_coeff]);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_cauchy(uint64_t* out, uint64_t N, uint64_t R, uint64_t fi, uint64_t fr,
+                         hipStream_t s) {
+	uint64_t total = N * R;
+	if (total == 0) return hipSuccess;
+	uint64_t blocks = (total + 255) / 256;
+	if (blocks > 65536) blocks = 65536;
+	hipLaunchKernelGGL(k_cauchy, dim3((unsigned)blocks), dim3(256), 0, s, out, N, R, fi, fr);
+	return hipGetLastError();
+}
+
+hipError_t launch_cauchy_points(uint64_t* out, const uint64_t* xs, uint64_t nx, const uint64_t* ys,
+                                uint64_t ny, hipStream_t s) {
+	uint64_t total = nx * ny;
+	if (total == 0) return hipSuccess;
+	uint64_t blocks = (total + 255) / 256;
+	if (blocks > 65536) blocks = 65536;
+	hipLaunchKernelGGL(k_cauchy_points, dim3((unsigned)blocks), dim3(256), 0, s, out, xs, nx, ys,
+	                   ny);
+	return hipGetLastError();
+}
+
+template <int RT, int WPT>
+static hipError_t launch_tab_rt(const MacArgs& a, uint64_t groups, hipStream_t s) {
+	constexpr int IB = (RT <= 8) ? 4 : 2;
+	const uint64_t tiles = (a.W + 256 * WPT - 1) / (256 * WPT);
+	const uint64_t rtiles = (a.R + RT - 1) / RT;
+	dim3 grid((unsigned)tiles, (unsigned)groups, (unsigned)rtiles);
+	hipLaunchKernelGGL((k_mac_tab<RT, WPT, IB>), grid, dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_mac_tab(const MacArgs& a, uint64_t groups, int rt, int wpt, hipStream_t s) {
+#define PARHIP_TAB_CASE(RT_, WPT_) \
+	if (rt == RT_ && wpt == WPT_) return launch_tab_rt<RT_, WPT_>(a, groups, s);
+	PARHIP_TAB_CASE(2, 2) PARHIP_TAB_CASE(2, 4)
+	PARHIP_TAB_CASE(4, 2) PARHIP_TAB_CASE(4, 4)
+	PARHIP_TAB_CASE(8, 2) PARHIP_TAB_CASE(8, 4)
+	PARHIP_TAB_CASE(16, 2) PARHIP_TAB_CASE(16, 4)
+#undef PARHIP_TAB_CASE
+	return hipErrorInvalidValue;
+}
+
+hipError_t launch_xor_reduce(const uint64_t* partial, uint64_t G, uint64_t R, uint64_t W,
+                             uint64_t* out_base, uint64_t out_stride, uint64_t* const* out_ptrs,
+                             int accumulate, hipStream_t s) {
+	uint64_t total = R * W;
+	if (total == 0) return hipSuccess;
+	uint64_t blocks = (total + 255) / 256;
+	if (blocks > 8192) blocks = 8192;
+	hipLaunchKernelGGL(k_xor_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, G, R, W,
+	                   out_base, out_stride, out_ptrs, accumulate);
+	return hipGetLastError();
+}
+
+hipError_t launch_region_mul_arr(uint64_t* out, const uint64_t* in, const uint64_t* coeff,
+                                 uint64_t len, uint64_t n_coeff, hipStream_t s) {
+	if (len == 0) return hipSuccess;
+	uint64_t blocks = (len + 255) / 256;
+	if (blocks > 16384) blocks = 16384;
+	hipLaunchKernelGGL(k_region_mul_arr, dim3((unsigned)blocks), dim3(256), 0, s, out, in, coeff,
+	                   len, n_coeff);
+	return hipGetLastError();
+}
+
+}  // namespace parhip

@@ -1,0 +1,37 @@
+// kernels.h -- launch interface between the C-ABI host code and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace parhip {
+
+// One contraction R[r][w] (^)= XOR_{i in [0,N)} coeff[r*coeff_ld + i] * in_i[w].
+// Inputs and outputs are either strided views of one allocation or arrays of
+// device pointers (scattered slices, e.g. the known blocks of a repair).
+struct MacArgs {
+	const uint64_t* in_base;
+	uint64_t in_stride;               // words between consecutive inputs
+	const uint64_t* const* in_ptrs;   // if non-null, overrides in_base/in_stride
+	uint64_t* out_base;
+	uint64_t out_stride;
+	uint64_t* const* out_ptrs;        // if non-null, overrides out_base/out_stride
+	const uint64_t* coeff;            // row-major, coeff[r*coeff_ld + i]
+	uint64_t coeff_ld;
+	uint64_t N, R, W;                 // inputs, outputs, words per slice
+	uint64_t in_per_group;            // split-K: inputs per work-group row
+	uint64_t* partial;                // if non-null: G x R x W partials, reduced afterwards
+	int accumulate;                   // XOR into existing outputs (when partial == nullptr)
+};
+
+hipError_t launch_cauchy(uint64_t* out, uint64_t N, uint64_t R, uint64_t fi, uint64_t fr,
+                         hipStream_t s);
+hipError_t launch_cauchy_points(uint64_t* out, const uint64_t* xs, uint64_t nx,
+                                const uint64_t* ys, uint64_t ny, hipStream_t s);
+hipError_t launch_mac_tab(const MacArgs& a, uint64_t groups, int rt, int wpt, hipStream_t s);
+hipError_t launch_xor_reduce(const uint64_t* partial, uint64_t G, uint64_t R, uint64_t W,
+                             uint64_t* out_base, uint64_t out_stride, uint64_t* const* out_ptrs,
+                             int accumulate, hipStream_t s);
+hipError_t launch_region_mul_arr(uint64_t* out, const uint64_t* in, const uint64_t* coeff,
+                                 uint64_t len, uint64_t n_coeff, hipStream_t s);
+
+}  // namespace parhip

@@ -1,0 +1,278 @@
+"""GF(2^64) parity: the gfx950 engine against the oracle, through the C ABI.
+
+Bit-exact is the bar everywhere (integer field arithmetic).  Small shapes are
+compared in full; the 1 GiB shapes through exact samples of word columns (each
+column of the contraction is independent, so a sampled column is a full
+check of that column) plus size-independent properties.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "par3-golden.bin")
+
+
+def rand_words(rng, *shape):
+    return rng.integers(0, 2 ** 64, size=shape, dtype=np.uint64)
+
+
+def test_golden_vector(gpu, oracle):
+    from parparpar_amd import binding
+    n, b = 5, 1024
+    inp = np.zeros((n, b // 8), dtype=np.uint64)
+    for f in range(n):
+        inp[f] = np.arange(b // 8, dtype=np.uint64) * 8 + f * b
+    out = np.zeros((2, b // 8), dtype=np.uint64)
+    binding.compute_recovery(inp, out, n, 2, b, 0, n, 0)
+    assert out.tobytes() == open(GOLDEN, "rb").read()
+    assert "%08X" % oracle.crc32(out.tobytes()) == "6A941C15"
+
+
+def test_kat(gpu):
+    from parparpar_amd import binding
+    out = np.zeros(1, dtype=np.uint64)
+    binding.compute_recovery(np.array([1, 2], dtype=np.uint64), out, 2, 1, 8, 0, 2)
+    assert int(out[0]) == 0x7ffffffffffffffa
+
+
+def test_config1_64x64k_r8(gpu, oracle):
+    # 64 input slices x 64 KiB of seeded random bytes, R=8, points (0, 64)
+    from parparpar_amd import binding
+    rng = np.random.default_rng(0xC1)
+    n, words, r = 64, 65536 // 8, 8
+    inp = np.frombuffer(rng.bytes(n * 65536), dtype=np.uint64).reshape(n, words).copy()
+    out = np.zeros((r, words), dtype=np.uint64)
+    binding.compute_recovery(inp, out, n, r, 65536, 0, n)
+    ref = oracle.recovery(inp, r, 0, n)
+    assert np.array_equal(out, ref)
+    # negative trap: a one-bit change in one input changes exactly that column
+    inp[17, 1234] ^= np.uint64(1 << 63)
+    out2 = np.zeros_like(out)
+    binding.compute_recovery(inp, out2, n, r, 65536, 0, n)
+    diff = np.nonzero((out2 != out).any(axis=0))[0]
+    assert diff.tolist() == [1234]
+
+
+SHAPES = [
+    # (N, words, R, firstInput, firstRecovery, accumulate)
+    (1, 1, 1, 0, 1, False),
+    (2, 3, 1, 0, 2, True),
+    (3, 64, 2, 5, 100, False),
+    (7, 513, 3, 0, 7, True),
+    (9, 1025, 5, 0, 9, False),
+    (17, 2048, 8, 1, 40, True),
+    (33, 4099, 9, 0, 33, False),
+    (64, 777, 16, 0, 64, True),
+    (10, 100, 17, 0, 10, False),
+    (50, 300, 33, 3, 1000, True),
+    (200, 64, 8, 0, 200, False),
+    (1000, 32, 4, 0, 1000, True),
+    (4, 8, 6, 2, 3, False),   # overlapping point ranges: zero denominator -> 1
+]
+
+
+@pytest.mark.parametrize("n,w,r,fi,fr,acc", SHAPES)
+def test_shapes(gpu, oracle, n, w, r, fi, fr, acc):
+    from parparpar_amd import binding
+    rng = np.random.default_rng(n * 7919 + w * 31 + r)
+    inp = rand_words(rng, n, w)
+    init = rand_words(rng, r, w)
+    out = init.copy()
+    binding.compute_recovery(inp, out, n, r, w * 8, fi, fr, 0, accumulate=acc)
+    expect = init.copy() if acc else np.zeros((r, w), dtype=np.uint64)
+    oracle.recovery_with_coeff(inp, oracle.cauchy(n, r, fi, fr), expect, acc)
+    assert np.array_equal(out, expect)
+
+
+def test_with_coeff_and_matrix(gpu, oracle):
+    from parparpar_amd import binding
+    rng = np.random.default_rng(5)
+    n, r, w = 37, 11, 200
+    m = binding.build_coefficient_matrix(n, r, 3, 500)
+    assert np.array_equal(m.reshape(r, n), oracle.cauchy(n, r, 3, 500))
+    big = binding.build_coefficient_matrix(1000, 40, 0, 1000)
+    assert np.array_equal(big.reshape(40, 1000), oracle.cauchy(1000, 40, 0, 1000))
+    co = rand_words(rng, r, n)  # arbitrary coefficients, including structure-free ones
+    co[0, :] = 0
+    co[1, :] = 1
+    inp = rand_words(rng, n, w)
+    out = np.zeros((r, w), dtype=np.uint64)
+    binding.compute_recovery_with_coeff(co, inp, out, n, r, w * 8)
+    assert np.array_equal(out, oracle.recovery_with_coeff(inp, co))
+    assert not out[0].any()
+    x = np.zeros(w, dtype=np.uint64)
+    for i in range(n):
+        x ^= inp[i]
+    assert np.array_equal(out[1], x)  # coefficient 1 -> plain XOR
+
+
+def test_chunked_accumulate_equals_single(gpu, oracle):
+    # lib/par3gen.js:2306-2316: compute_recovery_full per chunk with
+    # firstInput=chunkStart, firstRecovery=N, accumulate=true
+    from parparpar_amd import binding
+    rng = np.random.default_rng(6)
+    n, w, r = 100, 257, 8
+    inp = rand_words(rng, n, w)
+    single = np.zeros((r, w), dtype=np.uint64)
+    binding.compute_recovery_full(inp, single, n, r, w * 8, 0, n)
+    acc = np.zeros((r, w), dtype=np.uint64)
+    for s0 in range(0, n, 30):
+        chunk = np.ascontiguousarray(inp[s0:s0 + 30])
+        binding.compute_recovery_accumulate(chunk, acc, len(chunk), r, w * 8, s0, n)
+    assert np.array_equal(acc, single)
+    assert np.array_equal(single, oracle.recovery(inp, r, 0, n))
+
+
+def test_solve_and_reconstruct(gpu, oracle):
+    from parparpar_amd import binding
+    rng = np.random.default_rng(7)
+    for k, w in [(1, 5), (3, 64), (8, 1000), (20, 129)]:
+        A = oracle.cauchy(k, k, 0, 300)
+        rhs = rand_words(rng, k, w)
+        rc, A_o, x_o = oracle.solve_and_reconstruct(A, rhs)
+        A_g, x_g = A.copy(), rhs.copy()
+        assert binding.solve_and_reconstruct(A_g, x_g, k, w * 8) is True
+        assert np.array_equal(A_g, A_o) and np.array_equal(x_g, x_o)
+    # pivoting: zero leading entry forces a row swap
+    A = np.array([[0, 5, 7], [3, 0, 1], [2, 9, 0]], dtype=np.uint64)
+    rhs = rand_words(rng, 3, 17)
+    rc, A_o, x_o = oracle.solve_and_reconstruct(A, rhs)
+    A_g, x_g = A.copy(), rhs.copy()
+    assert binding.solve_and_reconstruct(A_g, x_g, 3, 17 * 8) is (rc == 0)
+    assert np.array_equal(A_g, A_o) and np.array_equal(x_g, x_o)
+    # singular: return false, leave the reference's partially eliminated state
+    A = np.array([[2, 3, 5], [4, 6, 10], [7, 1, 9]], dtype=np.uint64)
+    A[1] = [oracle.gf64_mul(int(v), 9) for v in A[0]]
+    rhs = rand_words(rng, 3, 9)
+    rc, A_o, x_o = oracle.solve_and_reconstruct(A, rhs)
+    assert rc == -1
+    A_g, x_g = A.copy(), rhs.copy()
+    assert binding.solve_and_reconstruct(A_g, x_g, 3, 9 * 8) is False
+    assert np.array_equal(A_g, A_o) and np.array_equal(x_g, x_o)
+
+
+def test_gf64_solve(gpu, oracle):
+    from parparpar_amd import binding
+    rng = np.random.default_rng(8)
+    A = oracle.cauchy(6, 6, 0, 50)
+    b = rand_words(rng, 6)
+    x = binding.gf64_solve(A, b, 6)
+    rc, _, xo = oracle.solve_and_reconstruct(A, b.reshape(6, 1))
+    assert np.array_equal(x, xo.reshape(6))
+    assert binding.gf64_solve(np.zeros((2, 2), dtype=np.uint64), b[:2], 2) is None
+
+
+def test_region_ops(gpu, oracle):
+    from parparpar_amd import binding
+    L = oracle.lib()
+    P = oracle.p64
+    enc = binding.Gf64Encoder()
+    rng = np.random.default_rng(9)
+    ln = 1031
+    x = rand_words(rng, ln)
+    co = rand_words(rng, 7)
+    # mul
+    out = np.zeros(ln, dtype=np.uint64)
+    enc.mul(out, x, int(co[0]), ln)
+    ref = np.zeros(ln, dtype=np.uint64)
+    L.or_gf64_region_mul(P(ref), P(x), ln, int(co[0]))
+    assert np.array_equal(out, ref)
+    # mul_arr cycling, n = 1, 3, 7 (gf64/test/test_gf64_region_mularr.c)
+    for nc in (1, 3, 7):
+        out = np.zeros(ln, dtype=np.uint64)
+        binding.mul_arr(out, x, co, ln, nc)
+        ref = np.zeros(ln, dtype=np.uint64)
+        L.or_gf64_region_mul_arr(P(ref), P(x), P(co), ln, nc)
+        assert np.array_equal(out, ref), nc
+    # muladd_arr: sum over all coefficients
+    init = rand_words(rng, ln)
+    out = init.copy()
+    enc.muladd_arr(out, x, co, ln, 5)
+    ref = init.copy()
+    L.or_gf64_region_muladd_arr(P(ref), P(x), P(co), ln, 5)
+    assert np.array_equal(out, ref)
+    # coupled / fused / 2d
+    G, K = 5, 4
+    ins = [rand_words(rng, ln) for _ in range(G)]
+    outs0 = [rand_words(rng, ln) for _ in range(K)]
+    c2d = rand_words(rng, K, G + 3)  # K_stride = G + 3
+    o = outs0[0].copy()
+    enc.coupled_muladd_arr(o, ins, np.ascontiguousarray(c2d[0, :G]), ln)
+    r = outs0[0].copy()
+    for g in range(G):
+        for i in range(ln):
+            r[i] ^= np.uint64(oracle.gf64_mul(int(ins[g][i]), int(c2d[0, g])))
+    assert np.array_equal(o, r)
+    outs = [a.copy() for a in outs0]
+    enc.fused_output_muladd_arr(outs, ins[0], c2d[:, 0].copy(), ln)
+    for k in range(K):
+        r = outs0[k].copy()
+        for i in range(ln):
+            r[i] ^= np.uint64(oracle.gf64_mul(int(ins[0][i]), int(c2d[k, 0])))
+        assert np.array_equal(outs[k], r)
+    outs = [a.copy() for a in outs0]
+    enc.two_d_muladd_arr(outs, ins, c2d, G + 3, ln)
+    co2 = np.ascontiguousarray(c2d[:, :G])
+    exp = np.stack(outs0).copy()
+    oracle.recovery_with_coeff(np.stack(ins), co2, exp, True)
+    for k in range(K):
+        assert np.array_equal(outs[k], exp[k])
+
+
+def test_input_errors(gpu):
+    from parparpar_amd import binding
+    with pytest.raises(ValueError):
+        binding.compute_recovery(np.zeros(2, np.uint64), np.zeros(2, np.uint64), 2, 1, 7, 0, 2)
+    # N == 0 / R == 0: silent no-op like the reference (src/par3_engine.cc:1069)
+    out = np.full(4, 5, dtype=np.uint64)
+    binding.compute_recovery(np.zeros(4, np.uint64), out, 0, 1, 32, 0, 0)
+    assert (out == 5).all()
+
+
+# ----------------------------------------------------------- device-resident, full size
+
+def _torch_words(rng, n, w, device="cuda"):
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(int(rng.integers(0, 2 ** 31)))
+    return torch.randint(-2 ** 63, 2 ** 63 - 1, (n, w), dtype=torch.int64, device=device, generator=g)
+
+
+@pytest.mark.parametrize("n,w,r", [(1024, 131072, 8), (262144, 512, 8)])
+def test_full_size_create_columns(gpu, oracle, n, w, r):
+    """1 GiB at R=8 in both geometries: exact check of sampled word columns."""
+    import torch
+    from parparpar_amd import device
+    rng = np.random.default_rng(n + w)
+    print("full-size create N=%d W=%d R=%d" % (n, w, r), flush=True)
+    inp = _torch_words(rng, n, w)
+    out = device.recovery(inp, r, 0, n)
+    torch.cuda.synchronize()
+    cols = np.unique(np.concatenate([[0, w - 1], rng.integers(0, w, size=14)])).astype(np.int64)
+    sample = inp[:, torch.as_tensor(cols, device="cuda")].cpu().numpy().view(np.uint64)
+    sample = np.ascontiguousarray(sample)
+    co = oracle.cauchy(n, r, 0, n)
+    ref = oracle.recovery_with_coeff(sample, co)
+    got = out[:, torch.as_tensor(cols, device="cuda")].cpu().numpy().view(np.uint64)
+    assert np.array_equal(got, ref)
+    # size-independent property: linearity over the whole 1 GiB input
+    out2 = device.recovery(inp, r, 0, n)
+    assert torch.equal(out, out2)  # deterministic
+    del inp, out, out2
+    torch.cuda.empty_cache()
+
+
+def test_device_strided_and_unaligned(gpu, oracle):
+    import torch
+    from parparpar_amd import device
+    rng = np.random.default_rng(21)
+    n, w, r = 13, 301, 5
+    base = _torch_words(rng, n, w + 3)
+    view = base[:, 1:w + 1]  # 8-byte aligned rows, odd stride: exercises the 64-bit load path
+    out = torch.zeros((r, w), dtype=torch.int64, device="cuda")
+    device.recovery(view, r, 0, n, out=out)
+    host = np.ascontiguousarray(view.cpu().numpy().view(np.uint64))
+    assert np.array_equal(out.cpu().numpy().view(np.uint64), oracle.recovery(host, r, 0, n))
