@@ -54,6 +54,10 @@ const char *parhip_last_error(void);
 int parhip_set_device(int device);
 /* Force a contraction strategy for this thread (PARHIP_METHOD_*). */
 int parhip_set_method(int method);
+/* Force the table-contraction launch shape for this thread (tuning; no
+ * counterpart in the reference). rt = outputs per work-group, wpt = words per
+ * thread, ib = inputs per LDS batch, groups = split-K rows; 0 = automatic. */
+int parhip_tune_override(int rt, int wpt, int ib, int64_t groups);
 /* Release cached device memory (coefficient cache, workspaces). */
 int parhip_release_cache(void);
 

@@ -51,6 +51,7 @@ _SIGS = [
     ("parhip_d_gf64_recovery", _int, [_vp, _sz, _sz, _vp, _sz, _sz, _sz, _u64, _u64, _int, _vp]),
     ("parhip_d_gf64_recovery_ptrs", _int, [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _int, _vp]),
     ("parhip_gf64_workspace_bytes", _sz, [_sz, _sz, _sz]),
+    ("parhip_tune_override", _int, [_int, _int, _int, ctypes.c_int64]),
     ("parhip_prof_enable", _int, [_int]),
     ("parhip_prof_reset", _int, []),
     ("parhip_prof_get", _int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),

@@ -259,15 +259,29 @@ static void prof_collect() {
 }
 
 // ------------------------------------------------------------------ contraction plan
+// Launch-shape override for tuning (parhip_tune_override); 0 fields = automatic.
+static thread_local int t_tune_rt = 0, t_tune_wpt = 0, t_tune_ib = 0;
+static thread_local int64_t t_tune_groups = 0;
+
 Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev) {
 	Plan p;
 	p.method = PARHIP_METHOD_TABLE;
 	p.rt = R >= 16 ? 16 : (R > 4 ? 8 : (R > 2 ? 4 : 2));
 	p.wpt = W >= 1024 ? 4 : 2;
+	p.ib = p.rt <= 8 ? 4 : 2;
+	if (t_tune_rt || t_tune_wpt || t_tune_ib) {
+		int rt = t_tune_rt ? t_tune_rt : p.rt, wpt = t_tune_wpt ? t_tune_wpt : p.wpt;
+		int ib = t_tune_ib ? t_tune_ib : p.ib;
+		if (mac_tab_supported(rt, wpt, ib)) {
+			p.rt = rt;
+			p.wpt = wpt;
+			p.ib = ib;
+		}
+	}
 	const uint64_t tile = 256ull * p.wpt;
 	const uint64_t tiles = (W + tile - 1) / tile;
 	const uint64_t rtiles = (R + p.rt - 1) / p.rt;
-	const int ib = p.rt <= 8 ? 4 : 2;
+	const int ib = p.ib;
 	// work-groups resident per CU: LDS-bound (IB*RT*2 KiB per group, 160 KiB per CU)
 	const int lds_kib = ib * p.rt * 2;
 	int per_cu = 160 / lds_kib;
@@ -277,6 +291,7 @@ Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev) {
 	uint64_t groups = 1;
 	const uint64_t base = tiles * rtiles;
 	if (base < target) groups = (target + base - 1) / base;
+	if (t_tune_groups > 0) groups = (uint64_t)t_tune_groups;
 	// at least 2*IB inputs per group, so table building stays amortised
 	const uint64_t max_groups = std::max<uint64_t>(1, N / (2 * ib));
 	groups = std::min(groups, max_groups);
@@ -290,7 +305,7 @@ Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev) {
 }
 
 size_t plan_workspace_bytes(const Plan& p, uint64_t R, uint64_t W) {
-	return p.groups > 1 ? (size_t)(p.groups * R * W * sizeof(uint64_t)) : 0;
+	return p.groups > 1 ? (size_t)(xor_reduce_scratch_words(p.groups, R, W) * sizeof(uint64_t)) : 0;
 }
 
 // Enqueue the full contraction described by `a` (partial/in_per_group are
@@ -310,7 +325,7 @@ int run_contraction(MacArgs a, hipStream_t s) {
 	hipError_t e;
 	{
 		ProfScope ps("mac_tab", s);
-		e = launch_mac_tab(a, p.groups, p.rt, p.wpt, s);
+		e = launch_mac_tab(a, p.groups, p.rt, p.wpt, p.ib, s);
 	}
 	if (e != hipSuccess) {
 		dev_free(ws, s);
@@ -369,6 +384,16 @@ int parhip_prof_get(const char* kernel, double* total_ms, uint64_t* launches) {
 	auto it = t_prof_tot.find(kernel);
 	*total_ms = it == t_prof_tot.end() ? 0.0 : it->second.first;
 	*launches = it == t_prof_tot.end() ? 0 : it->second.second;
+	return PARHIP_OK;
+}
+
+int parhip_tune_override(int rt, int wpt, int ib, int64_t groups) {
+	if ((rt || wpt || ib) && !mac_tab_supported(rt ? rt : 8, wpt ? wpt : 4, ib ? ib : 4))
+		return fail(PARHIP_ERR_ARG, "tune_override: unsupported (rt=%d wpt=%d ib=%d)", rt, wpt, ib);
+	t_tune_rt = rt;
+	t_tune_wpt = wpt;
+	t_tune_ib = ib;
+	t_tune_groups = groups;
 	return PARHIP_OK;
 }
 

@@ -9,7 +9,7 @@ namespace parhip {
 
 struct Plan {
 	int method;            // PARHIP_METHOD_TABLE / PARHIP_METHOD_MFMA
-	int rt, wpt;           // outputs per work-group, words per thread
+	int rt, wpt, ib;       // outputs per work-group, words per thread, inputs per LDS batch
 	uint64_t groups;       // split-K work-group rows
 	uint64_t in_per_group; // inputs per row
 };

@@ -52,6 +52,32 @@ __global__ void k_cauchy_points(uint64_t* __restrict__ out, const uint64_t* __re
 // --------------------------------------------------------------------------
 // Nibble-table contraction.
 // --------------------------------------------------------------------------
+// Global-memory views: the engine only hands device (HBM) pointers to the
+// kernels; loading through address space 1 keeps these off the flat path
+// (flat loads share the LDS wait counter and serialise against ds_read).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(1))) const uint64_t gu64;
+typedef __attribute__((address_space(1))) const ulonglong2 gu128;
+__device__ __forceinline__ uint64_t gload(const uint64_t* p) { return *(gu64*)p; }
+__device__ __forceinline__ ulonglong2 gload2(const uint64_t* p) {
+	gu128* q = (gu128*)p;
+	ulonglong2 v;
+	v.x = q->x;
+	v.y = q->y;
+	return v;
+}
+#else
+__device__ __forceinline__ uint64_t gload(const uint64_t* p) { return *p; }
+__device__ __forceinline__ ulonglong2 gload2(const uint64_t* p) {
+	return *reinterpret_cast<const ulonglong2*>(p);
+}
+#endif
+
+// a ^ b ^ c in one VALU op (gfx950 v_bitop3_b32, truth table 0x96)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+	return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 template <int RT, int WPT, int IB>
 __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 	static_assert(RT % 2 == 0 && RT >= 2 && RT <= 16, "RT");
@@ -69,16 +95,17 @@ __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 	uint64_t i_end = i_begin + a.in_per_group;
 	if (i_end > a.N) i_end = a.N;
 
-	// word positions owned by this thread: pairs (2*tid + 512*j, +1)
-	uint64_t acc[WPT][RT];
+	// word positions owned by this thread: pairs (2*tid + 512*j, +1);
+	// acc[j][p] = (lo, hi) of output 2p and (lo, hi) of output 2p+1
+	uint4 acc[WPT][NP];
 #pragma unroll
 	for (int j = 0; j < WPT; j++)
 #pragma unroll
-		for (int r = 0; r < RT; r++) acc[j][r] = 0;
+		for (int p = 0; p < NP; p++) acc[j][p] = make_uint4(0, 0, 0, 0);
 
 	for (uint64_t ib0 = i_begin; ib0 < i_end; ib0 += IB) {
 		// ---- issue data loads for this batch (in flight during the table build)
-		uint64_t d[IB][WPT];
+		uint2 d[IB][WPT];
 #pragma unroll
 		for (int ib = 0; ib < IB; ib++) {
 			uint64_t i = ib0 + ib;
@@ -90,16 +117,16 @@ __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 				uint64_t v0 = 0, v1 = 0;
 				if (src) {
 					if (w + 1 < a.W && ((reinterpret_cast<uintptr_t>(src) & 15u) == 0)) {
-						const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(src + w);
+						const ulonglong2 v = gload2(src + w);
 						v0 = v.x;
 						v1 = v.y;
 					} else {
-						if (w < a.W) v0 = src[w];
-						if (w + 1 < a.W) v1 = src[w + 1];
+						if (w < a.W) v0 = gload(src + w);
+						if (w + 1 < a.W) v1 = gload(src + w + 1);
 					}
 				}
-				d[ib][2 * j] = v0;
-				d[ib][2 * j + 1] = v1;
+				d[ib][2 * j] = make_uint2((uint32_t)v0, (uint32_t)(v0 >> 32));
+				d[ib][2 * j + 1] = make_uint2((uint32_t)v1, (uint32_t)(v1 >> 32));
 			}
 		}
 		// ---- build the split tables for IB inputs x RT outputs
@@ -112,8 +139,8 @@ __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 			uint64_t c0 = 0, c1 = 0;
 			if (i < i_end) {
 				const uint64_t ra = r0 + 2 * p, rb = ra + 1;
-				if (ra < a.R) c0 = a.coeff[ra * a.coeff_ld + i];
-				if (rb < a.R) c1 = a.coeff[rb * a.coeff_ld + i];
+				if (ra < a.R) c0 = gload(a.coeff + ra * a.coeff_ld + i);
+				if (rb < a.R) c1 = gload(a.coeff + rb * a.coeff_ld + i);
 			}
 			uint64_t b0[4], b1[4];
 #pragma unroll
@@ -141,20 +168,33 @@ __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 			}
 		}
 		__syncthreads();
-		// ---- lookups
+		// ---- lookups: four nibble positions per step, folded with 3-input XORs
 #pragma unroll
 		for (int ib = 0; ib < IB; ib++) {
 #pragma unroll
 			for (int j = 0; j < WPT; j++) {
-				const uint64_t x = d[ib][j];
+				const uint2 x = d[ib][j];
 #pragma unroll
-				for (int k = 0; k < 16; k++) {
-					const unsigned n = (unsigned)(x >> (4 * k)) & 15u;
+				for (int kq = 0; kq < 16; kq += 4) {
+					unsigned n[4];
+#pragma unroll
+					for (int u = 0; u < 4; u++) {
+						const int k = kq + u;
+						const uint32_t half = k < 8 ? x.x : x.y;
+						n[u] = (half >> (4 * (k & 7))) & 15u;
+					}
 #pragma unroll
 					for (int p = 0; p < NP; p++) {
-						const uint4 v = tab[ib][p][k][n];
-						acc[j][2 * p] ^= ((uint64_t)v.y << 32) | v.x;
-						acc[j][2 * p + 1] ^= ((uint64_t)v.w << 32) | v.z;
+						const uint4 v0 = tab[ib][p][kq + 0][n[0]];
+						const uint4 v1 = tab[ib][p][kq + 1][n[1]];
+						const uint4 v2 = tab[ib][p][kq + 2][n[2]];
+						const uint4 v3 = tab[ib][p][kq + 3][n[3]];
+						uint4 c = acc[j][p];
+						c.x = xor3(xor3(c.x, v0.x, v1.x), v2.x, v3.x);
+						c.y = xor3(xor3(c.y, v0.y, v1.y), v2.y, v3.y);
+						c.z = xor3(xor3(c.z, v0.z, v1.z), v2.z, v3.z);
+						c.w = xor3(xor3(c.w, v0.w, v1.w), v2.w, v3.w);
+						acc[j][p] = c;
 					}
 				}
 			}
@@ -175,29 +215,50 @@ __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 			dst = a.out_ptrs ? a.out_ptrs[rr] : a.out_base + rr * a.out_stride;
 			xr = a.accumulate != 0;
 		}
+		const int p = r >> 1;
 #pragma unroll
 		for (int j = 0; j < WPT / 2; j++) {
 			const uint64_t w = w0 + 2 * tid + 512 * j;
+			const uint4 c0 = acc[2 * j][p], c1 = acc[2 * j + 1][p];
+			uint64_t o0 = (r & 1) ? (((uint64_t)c0.w << 32) | c0.z) : (((uint64_t)c0.y << 32) | c0.x);
+			uint64_t o1 = (r & 1) ? (((uint64_t)c1.w << 32) | c1.z) : (((uint64_t)c1.y << 32) | c1.x);
 			if (w + 1 < a.W && ((reinterpret_cast<uintptr_t>(dst) & 15u) == 0)) {
 				ulonglong2* q = reinterpret_cast<ulonglong2*>(dst + w);
-				ulonglong2 v;
-				v.x = acc[2 * j][r];
-				v.y = acc[2 * j + 1][r];
 				if (xr) {
 					const ulonglong2 o = *q;
-					v.x ^= o.x;
-					v.y ^= o.y;
+					o0 ^= o.x;
+					o1 ^= o.y;
 				}
+				ulonglong2 v;
+				v.x = o0;
+				v.y = o1;
 				*q = v;
 			} else {
-				if (w < a.W) dst[w] = xr ? (dst[w] ^ acc[2 * j][r]) : acc[2 * j][r];
-				if (w + 1 < a.W) dst[w + 1] = xr ? (dst[w + 1] ^ acc[2 * j + 1][r]) : acc[2 * j + 1][r];
+				if (w < a.W) dst[w] = xr ? (dst[w] ^ o0) : o0;
+				if (w + 1 < a.W) dst[w + 1] = xr ? (dst[w + 1] ^ o1) : o1;
 			}
 		}
 	}
 }
 
-// out[r][w] = (accumulate ? out[r][w] : 0) ^ XOR_g partial[g][r][w]
+// Split-K partial reduction.  Pass: dst[h][r][w] = XOR_{g in chunk h} src[g][r][w],
+// F partials per chunk, one thread per (h, r, w) -- plenty of parallelism even
+// when R*W is small and G is large (many small slices).
+__global__ void k_xor_fold(const uint64_t* __restrict__ src, uint64_t G, uint64_t RW, uint64_t F,
+                           uint64_t* __restrict__ dst) {
+	const uint64_t H = (G + F - 1) / F;
+	const uint64_t total = H * RW;
+	for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+	     idx += (uint64_t)gridDim.x * blockDim.x) {
+		const uint64_t h = idx / RW, e = idx - h * RW;
+		const uint64_t g0 = h * F, g1 = (g0 + F < G) ? g0 + F : G;
+		uint64_t s = 0;
+		for (uint64_t g = g0; g < g1; g++) s ^= src[g * RW + e];
+		dst[idx] = s;
+	}
+}
+
+// Final: out[r][w] = (accumulate ? out[r][w] : 0) ^ XOR_g partial[g][r][w] (G small)
 __global__ void k_xor_reduce(const uint64_t* __restrict__ partial, uint64_t G, uint64_t R,
                              uint64_t W, uint64_t* out_base, uint64_t out_stride,
                              uint64_t* const* out_ptrs, int accumulate) {
@@ -245,9 +306,8 @@ hipError_t launch_cauchy_points(uint64_t* out, const uint64_t* xs, uint64_t nx, 
 	return hipGetLastError();
 }
 
-template <int RT, int WPT>
+template <int RT, int WPT, int IB>
 static hipError_t launch_tab_rt(const MacArgs& a, uint64_t groups, hipStream_t s) {
-	constexpr int IB = (RT <= 8) ? 4 : 2;
 	const uint64_t tiles = (a.W + 256 * WPT - 1) / (256 * WPT);
 	const uint64_t rtiles = (a.R + RT - 1) / RT;
 	dim3 grid((unsigned)tiles, (unsigned)groups, (unsigned)rtiles);
@@ -255,25 +315,61 @@ static hipError_t launch_tab_rt(const MacArgs& a, uint64_t groups, hipStream_t s
 	return hipGetLastError();
 }
 
-hipError_t launch_mac_tab(const MacArgs& a, uint64_t groups, int rt, int wpt, hipStream_t s) {
-#define PARHIP_TAB_CASE(RT_, WPT_) \
-	if (rt == RT_ && wpt == WPT_) return launch_tab_rt<RT_, WPT_>(a, groups, s);
-	PARHIP_TAB_CASE(2, 2) PARHIP_TAB_CASE(2, 4)
-	PARHIP_TAB_CASE(4, 2) PARHIP_TAB_CASE(4, 4)
-	PARHIP_TAB_CASE(8, 2) PARHIP_TAB_CASE(8, 4)
-	PARHIP_TAB_CASE(16, 2) PARHIP_TAB_CASE(16, 4)
+hipError_t launch_mac_tab(const MacArgs& a, uint64_t groups, int rt, int wpt, int ib,
+                          hipStream_t s) {
+#define PARHIP_TAB_CASE(RT_, WPT_, IB_) \
+	if (rt == RT_ && wpt == WPT_ && ib == IB_) return launch_tab_rt<RT_, WPT_, IB_>(a, groups, s);
+	PARHIP_TAB_CASE(2, 2, 4) PARHIP_TAB_CASE(2, 4, 4)
+	PARHIP_TAB_CASE(4, 2, 4) PARHIP_TAB_CASE(4, 4, 4) PARHIP_TAB_CASE(4, 2, 2) PARHIP_TAB_CASE(4, 4, 2)
+	PARHIP_TAB_CASE(8, 2, 4) PARHIP_TAB_CASE(8, 4, 4) PARHIP_TAB_CASE(8, 2, 2) PARHIP_TAB_CASE(8, 4, 2)
+	PARHIP_TAB_CASE(8, 2, 1) PARHIP_TAB_CASE(8, 4, 1)
+	PARHIP_TAB_CASE(16, 2, 2) PARHIP_TAB_CASE(16, 4, 2) PARHIP_TAB_CASE(16, 2, 1)
 #undef PARHIP_TAB_CASE
 	return hipErrorInvalidValue;
 }
 
-hipError_t launch_xor_reduce(const uint64_t* partial, uint64_t G, uint64_t R, uint64_t W,
+bool mac_tab_supported(int rt, int wpt, int ib) {
+	static const int cfg[][3] = {{2, 2, 4}, {2, 4, 4}, {4, 2, 4}, {4, 4, 4}, {4, 2, 2}, {4, 4, 2},
+	                             {8, 2, 4}, {8, 4, 4}, {8, 2, 2}, {8, 4, 2}, {8, 2, 1}, {8, 4, 1},
+	                             {16, 2, 2}, {16, 4, 2}, {16, 2, 1}};
+	for (auto& c : cfg)
+		if (c[0] == rt && c[1] == wpt && c[2] == ib) return true;
+	return false;
+}
+
+// Reduce G partials into the outputs.  `partial` holds G x R x W words followed
+// by a tail of ceil(G/16) x R x W words (xor_reduce_scratch_words); folding
+// passes of 16 ping-pong between the two regions until at most 16 remain, then
+// k_xor_reduce writes (or XORs into) the outputs.  Stream order separates the
+// passes, so a region is only rewritten after the pass reading it finished.
+uint64_t xor_reduce_scratch_words(uint64_t G, uint64_t R, uint64_t W) {
+	const uint64_t F = 16;
+	return (G + (G > F ? (G + F - 1) / F : 0)) * R * W;
+}
+
+hipError_t launch_xor_reduce(uint64_t* partial, uint64_t G, uint64_t R, uint64_t W,
                              uint64_t* out_base, uint64_t out_stride, uint64_t* const* out_ptrs,
                              int accumulate, hipStream_t s) {
-	uint64_t total = R * W;
-	if (total == 0) return hipSuccess;
-	uint64_t blocks = (total + 255) / 256;
-	if (blocks > 8192) blocks = 8192;
-	hipLaunchKernelGGL(k_xor_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, G, R, W,
+	const uint64_t RW = R * W;
+	if (RW == 0) return hipSuccess;
+	const uint64_t F = 16;
+	uint64_t* front = partial;
+	uint64_t* tail = partial + G * RW;
+	uint64_t* src = front;
+	while (G > F) {
+		uint64_t* dst = (src == front) ? tail : front;
+		const uint64_t H = (G + F - 1) / F;
+		uint64_t blocks = (H * RW + 255) / 256;
+		if (blocks > 65536) blocks = 65536;
+		hipLaunchKernelGGL(k_xor_fold, dim3((unsigned)blocks), dim3(256), 0, s, src, G, RW, F, dst);
+		hipError_t e = hipGetLastError();
+		if (e != hipSuccess) return e;
+		src = dst;
+		G = H;
+	}
+	uint64_t blocks = (RW + 255) / 256;
+	if (blocks > 16384) blocks = 16384;
+	hipLaunchKernelGGL(k_xor_reduce, dim3((unsigned)blocks), dim3(256), 0, s, src, G, R, W,
 	                   out_base, out_stride, out_ptrs, accumulate);
 	return hipGetLastError();
 }

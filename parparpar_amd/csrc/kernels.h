@@ -27,8 +27,11 @@ hipError_t launch_cauchy(uint64_t* out, uint64_t N, uint64_t R, uint64_t fi, uin
                          hipStream_t s);
 hipError_t launch_cauchy_points(uint64_t* out, const uint64_t* xs, uint64_t nx,
                                 const uint64_t* ys, uint64_t ny, hipStream_t s);
-hipError_t launch_mac_tab(const MacArgs& a, uint64_t groups, int rt, int wpt, hipStream_t s);
-hipError_t launch_xor_reduce(const uint64_t* partial, uint64_t G, uint64_t R, uint64_t W,
+hipError_t launch_mac_tab(const MacArgs& a, uint64_t groups, int rt, int wpt, int ib,
+                          hipStream_t s);
+bool mac_tab_supported(int rt, int wpt, int ib);
+uint64_t xor_reduce_scratch_words(uint64_t G, uint64_t R, uint64_t W);
+hipError_t launch_xor_reduce(uint64_t* partial, uint64_t G, uint64_t R, uint64_t W,
                              uint64_t* out_base, uint64_t out_stride, uint64_t* const* out_ptrs,
                              int accumulate, hipStream_t s);
 hipError_t launch_region_mul_arr(uint64_t* out, const uint64_t* in, const uint64_t* coeff,
