@@ -52,6 +52,22 @@ _SIGS = [
     ("parhip_d_gf64_recovery_ptrs", _int, [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _int, _vp]),
     ("parhip_gf64_workspace_bytes", _sz, [_sz, _sz, _sz]),
     ("parhip_tune_override", _int, [_int, _int, _int, ctypes.c_int64]),
+    ("parhip_gf16_coeff", ctypes.c_uint16, [ctypes.c_uint16, ctypes.c_uint16]),
+    ("parhip_gf16_mul", ctypes.c_uint16, [ctypes.c_uint16, ctypes.c_uint16]),
+    ("parhip_d_gf16_coeffs", _int, [_vp, _vp, _sz, _vp, _sz, _vp]),
+    ("parhip_d_gf16_muladd", _int, [_vp, _sz, _sz, _vp, _sz, _sz, _sz, _vp, _int, _vp]),
+    ("parhip_par2_create", _vp, [_int, _sz, ctypes.c_uint]),
+    ("parhip_par2_destroy", None, [_vp]),
+    ("parhip_par2_set_recovery_slices", _int, [_vp, ctypes.c_uint, _vp]),
+    ("parhip_par2_set_current_slice_size", _int, [_vp, _sz]),
+    ("parhip_par2_can_add", _int, [_vp]),
+    ("parhip_par2_add_input", _int, [_vp, _vp, _sz, ctypes.c_uint16, _int]),
+    ("parhip_par2_add_input_coeffs", _int, [_vp, _vp, _sz, _vp, _int]),
+    ("parhip_par2_flush", _int, [_vp]),
+    ("parhip_par2_end_input", _int, [_vp]),
+    ("parhip_par2_get_output", _int, [_vp, ctypes.c_uint, _vp]),
+    ("parhip_par2_discard_output", _int, [_vp]),
+    ("parhip_par2_input_batch_size", _int, [_vp]),
     ("parhip_prof_enable", _int, [_int]),
     ("parhip_prof_reset", _int, []),
     ("parhip_prof_get", _int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),

@@ -1,0 +1,402 @@
+// controller_hip.cpp -- PAR2ProcHIP (see controller_hip.h) and its C ABI.
+#include "controller_hip.h"
+
+#include <algorithm>
+#include <cstring>
+
+#include "../../include/parhip.h"
+#include "engine_internal.h"
+#include "gf16_host.h"
+#include "kernels.h"
+
+namespace parhip {
+
+int run_gf16_contraction(Gf16Args a, hipStream_t s);
+
+static std::future<void> ready_future() {
+	std::promise<void> p;
+	p.set_value();
+	return p.get_future();
+}
+
+PAR2ProcHIP::PAR2ProcHIP(int device, unsigned stagingAreas) : dev(device) {
+	staging.resize(stagingAreas ? stagingAreas : 2);
+	hipError_t e = hipSetDevice(dev);
+	if (e == hipSuccess) e = hipStreamCreateWithFlags(&strm, hipStreamNonBlocking);
+	if (e != hipSuccess) setErr("stream", e);
+}
+
+PAR2ProcHIP::~PAR2ProcHIP() { deinit(); }
+
+void PAR2ProcHIP::setErr(const char* what, hipError_t e) {
+	err = std::string(what) + ": " + hipGetErrorString(e);
+}
+
+bool PAR2ProcHIP::init(unsigned batch) {
+	inputBatchSize = batch ? batch : 64;
+	minInBatchSize = inputBatchSize;
+	currentStagingArea = currentStagingInputs = 0;
+	statBatchesStarted = 0;
+	processingAdd = false;
+	return allocStaging() && allocOutputs();
+}
+
+void PAR2ProcHIP::freeStaging() {
+	hipSetDevice(dev);
+	for (auto& st : staging) {
+		if (st.done) hipEventSynchronize(st.done);
+		if (st.hsrc) hipHostFree(st.hsrc);
+		if (st.hcoef) hipHostFree(st.hcoef);
+		if (st.dsrc) hipFree(st.dsrc);
+		if (st.dcoef) hipFree(st.dcoef);
+		if (st.done) hipEventDestroy(st.done);
+		st = Staging();
+	}
+}
+
+bool PAR2ProcHIP::allocStaging() {
+	freeStaging();
+	if (!allocWords || !inputBatchSize) return true;
+	hipSetDevice(dev);
+	const size_t nOut = std::max<size_t>(outputExponents.size(), 1);
+	for (auto& st : staging) {
+		const size_t sb = (size_t)inputBatchSize * strideWords * sizeof(uint16_t);
+		const size_t cb = nOut * inputBatchSize * sizeof(uint16_t);
+		hipError_t e;
+		if ((e = hipHostMalloc((void**)&st.hsrc, sb, 0)) != hipSuccess ||
+		    (e = hipHostMalloc((void**)&st.hcoef, cb, 0)) != hipSuccess ||
+		    (e = hipMalloc((void**)&st.dsrc, sb)) != hipSuccess ||
+		    (e = hipMalloc((void**)&st.dcoef, cb)) != hipSuccess ||
+		    (e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming)) != hipSuccess) {
+			setErr("staging allocation", e);
+			return false;
+		}
+		std::memset(st.hsrc, 0, sb);
+	}
+	return true;
+}
+
+bool PAR2ProcHIP::allocOutputs() {
+	hipSetDevice(dev);
+	if (dout) {
+		hipStreamSynchronize(strm);
+		hipFree(dout);
+		dout = nullptr;
+	}
+	if (!allocWords || outputExponents.empty()) return true;
+	hipError_t e = hipMalloc((void**)&dout, outputExponents.size() * strideWords * sizeof(uint16_t));
+	if (e != hipSuccess) {
+		setErr("output allocation", e);
+		return false;
+	}
+	allocOutputsCount = outputExponents.size();
+	return true;
+}
+
+void PAR2ProcHIP::setSliceSize(size_t size) {
+	sliceSize = currentSliceSize = size;
+	words = (size + 1) / 2 + 1;  // payload words (odd tail byte padded) + checksum word
+	allocWords = words;
+	strideWords = (words + 3) & ~(size_t)3;  // 8-byte aligned rows
+	allocStaging();
+	allocOutputs();
+}
+
+bool PAR2ProcHIP::setCurrentSliceSize(size_t size) {
+	const size_t w = (size + 1) / 2 + 1;
+	if (w > allocWords) {
+		setSliceSize(size);
+		return ok();
+	}
+	currentSliceSize = size;
+	words = w;
+	return true;
+}
+
+bool PAR2ProcHIP::setRecoverySlices(unsigned numSlices, const uint16_t* exponents) {
+	hipSetDevice(dev);
+	hipStreamSynchronize(strm);
+	outputExponents.assign(numSlices, 1);  // no exponents: custom coefficients (controller_cpu.cpp:176)
+	if (exponents) std::copy(exponents, exponents + numSlices, outputExponents.begin());
+	processingAdd = false;
+	bool okay = true;
+	if (numSlices > allocOutputsCount || !dout) okay = allocOutputs() && allocStaging();
+	return okay && ok();
+}
+
+PAR2ProcBackendAddResult PAR2ProcHIP::canAdd() const {
+	const Staging& st = staging[currentStagingArea];
+	if (st.active && hipEventQuery(st.done) == hipErrorNotReady) return PROC_ADD_FULL;
+	for (const auto& s : staging)
+		if (s.active && hipEventQuery(s.done) == hipErrorNotReady) return PROC_ADD_OK_BUSY;
+	return PROC_ADD_OK;
+}
+
+void PAR2ProcHIP::waitForAdd() {
+	Staging& st = staging[currentStagingArea];
+	if (st.active) {
+		hipEventSynchronize(st.done);
+		st.active = false;
+	}
+}
+
+void PAR2ProcHIP::setCoeffs(Staging& st, unsigned idx, uint16_t inputNum) {
+	for (size_t e = 0; e < outputExponents.size(); e++)
+		st.hcoef[e * inputBatchSize + idx] = gf16_host_coeff(inputNum, outputExponents[e]);
+}
+
+void PAR2ProcHIP::setCoeffs(Staging& st, unsigned idx, const uint16_t* coeffs) {
+	for (size_t e = 0; e < outputExponents.size(); e++)
+		st.hcoef[e * inputBatchSize + idx] = coeffs[e];
+}
+
+template <class T>
+std::future<void> PAR2ProcHIP::addInputT(const void* buffer, size_t size, T numOrCoeffs,
+                                         bool flushNow) {
+	waitForAdd();  // the area we are about to fill must not be in flight
+	Staging& st = staging[currentStagingArea];
+	uint16_t* slot = st.hsrc + (size_t)currentStagingInputs * strideWords;
+	const size_t payloadWords = words - 1;
+	if (size > payloadWords * 2) size = payloadWords * 2;
+	std::memcpy(slot, buffer, size);
+	std::memset(reinterpret_cast<char*>(slot) + size, 0, (words * 2) - size);
+	// linear checksum word: XOR of all payload words (GF(2^16)-linear, so the
+	// muladd carries it along: cksum(out) == XOR_i c_i * cksum(in_i))
+	uint64_t x = 0;
+	size_t w = 0;
+	for (; w + 4 <= payloadWords; w += 4) {
+		uint64_t v;
+		std::memcpy(&v, slot + w, 8);
+		x ^= v;
+	}
+	uint16_t ck = (uint16_t)(x ^ (x >> 16) ^ (x >> 32) ^ (x >> 48));
+	for (; w < payloadWords; w++) ck ^= slot[w];
+	slot[payloadWords] = ck;
+	setCoeffs(st, currentStagingInputs, numOrCoeffs);
+	currentStagingInputs++;
+	if (currentStagingInputs >= inputBatchSize || flushNow) flush();
+	return ready_future();  // the caller's buffer has been copied: reusable now
+}
+
+std::future<void> PAR2ProcHIP::addInput(const void* buffer, size_t size, uint16_t inputNum,
+                                        bool flushNow) {
+	return addInputT(buffer, size, inputNum, flushNow);
+}
+
+std::future<void> PAR2ProcHIP::addInput(const void* buffer, size_t size, const uint16_t* coeffs,
+                                        bool flushNow) {
+	return addInputT(buffer, size, coeffs, flushNow);
+}
+
+void PAR2ProcHIP::dummyInput(uint16_t inputNum, bool flushNow) {
+	waitForAdd();
+	setCoeffs(staging[currentStagingArea], currentStagingInputs, inputNum);
+	currentStagingInputs++;
+	if (currentStagingInputs >= inputBatchSize || flushNow) flush();
+}
+
+bool PAR2ProcHIP::fillInput(const void* buffer) {
+	for (auto& st : staging) {
+		if (st.active) hipEventSynchronize(st.done);
+		st.active = false;
+		for (unsigned i = 0; i < inputBatchSize; i++) {
+			uint16_t* slot = st.hsrc + (size_t)i * strideWords;
+			std::memcpy(slot, buffer, (words - 1) * 2);
+			slot[words - 1] = 0;
+			uint16_t ck = 0;
+			for (size_t w = 0; w + 1 < words; w++) ck ^= slot[w];
+			slot[words - 1] = ck;
+		}
+	}
+	return true;
+}
+
+void PAR2ProcHIP::flush() {
+	if (!currentStagingInputs) return;
+	run_kernel(currentStagingArea, currentStagingInputs);
+	currentStagingArea = (currentStagingArea + 1) % staging.size();
+	currentStagingInputs = 0;
+}
+
+void PAR2ProcHIP::run_kernel(unsigned area, unsigned numInputs) {
+	Staging& st = staging[area];
+	hipSetDevice(dev);
+	hipError_t e = hipMemcpyAsync(st.dsrc, st.hsrc, (size_t)numInputs * strideWords * 2,
+	                              hipMemcpyHostToDevice, strm);
+	if (e == hipSuccess)
+		e = hipMemcpyAsync(st.dcoef, st.hcoef, outputExponents.size() * inputBatchSize * 2,
+		                   hipMemcpyHostToDevice, strm);
+	if (e != hipSuccess) {
+		setErr("staging copy", e);
+		return;
+	}
+	Gf16Args a{};
+	a.in_base = st.dsrc;
+	a.in_stride = strideWords;
+	a.out_base = dout;
+	a.out_stride = strideWords;
+	a.coeff = st.dcoef;
+	a.coeff_ld = inputBatchSize;
+	a.N = numInputs;
+	a.R = outputExponents.size();
+	a.W = words;
+	a.accumulate = processingAdd ? 1 : 0;
+	if (run_gf16_contraction(a, strm) != PARHIP_OK) err = "gf16 contraction launch failed";
+	processingAdd = true;
+	hipEventRecord(st.done, strm);
+	st.active = true;
+	statBatchesStarted++;
+}
+
+std::future<void> PAR2ProcHIP::endInput() {
+	flush();
+	hipStreamSynchronize(strm);
+	for (auto& st : staging) st.active = false;
+	return ready_future();
+}
+
+std::future<bool> PAR2ProcHIP::getOutput(unsigned index, void* output) {
+	std::promise<bool> pr;
+	hipSetDevice(dev);
+	hipStreamSynchronize(strm);
+	if (index >= outputExponents.size()) {
+		pr.set_value(false);
+		return pr.get_future();
+	}
+	if (!processingAdd) {  // nothing added in this pass: all-zero recovery
+		std::memset(output, 0, currentSliceSize);
+		pr.set_value(true);
+		return pr.get_future();
+	}
+	std::vector<uint16_t> tmp(words);
+	hipError_t e = hipMemcpy(tmp.data(), dout + (size_t)index * strideWords, words * 2,
+	                         hipMemcpyDeviceToHost);
+	if (e != hipSuccess) {
+		setErr("output copy", e);
+		pr.set_value(false);
+		return pr.get_future();
+	}
+	uint16_t ck = 0;
+	for (size_t w = 0; w + 1 < words; w++) ck ^= tmp[w];
+	std::memcpy(output, tmp.data(), currentSliceSize);
+	pr.set_value(ck == tmp[words - 1]);
+	return pr.get_future();
+}
+
+void PAR2ProcHIP::freeProcessingMem() {
+	hipSetDevice(dev);
+	hipStreamSynchronize(strm);
+	if (dout) hipFree(dout);
+	dout = nullptr;
+	allocOutputsCount = 0;
+}
+
+void PAR2ProcHIP::deinit() {
+	if (strm) hipStreamSynchronize(strm);
+	freeStaging();
+	freeProcessingMem();
+	if (strm) hipStreamDestroy(strm);
+	strm = nullptr;
+}
+
+}  // namespace parhip
+
+using namespace parhip;
+
+// ============================================================ C ABI
+struct parhip_par2 {
+	PAR2ProcHIP* be;
+};
+
+extern "C" {
+
+parhip_par2* parhip_par2_create(int device, size_t sliceSize, unsigned inputBatchSize) {
+	auto* h = new (std::nothrow) parhip_par2;
+	if (!h) return nullptr;
+	h->be = new (std::nothrow) PAR2ProcHIP(device, 2);
+	if (!h->be) {
+		delete h;
+		return nullptr;
+	}
+	h->be->setSliceSize(sliceSize);
+	if (!h->be->init(inputBatchSize) || !h->be->ok()) {
+		fail(PARHIP_ERR_DEVICE, "par2_create: %s", h->be->lastError().c_str());
+		delete h->be;
+		delete h;
+		return nullptr;
+	}
+	return h;
+}
+
+void parhip_par2_destroy(parhip_par2* h) {
+	if (!h) return;
+	delete h->be;
+	delete h;
+}
+
+static int be_status(parhip_par2* h) {
+	if (!h->be->ok()) return fail(PARHIP_ERR_DEVICE, "par2: %s", h->be->lastError().c_str());
+	return PARHIP_OK;
+}
+
+int parhip_par2_set_recovery_slices(parhip_par2* h, unsigned numSlices, const uint16_t* exponents) {
+	if (!h) return fail(PARHIP_ERR_ARG, "par2: null handle");
+	if (!h->be->setRecoverySlices(numSlices, exponents)) return be_status(h) ? be_status(h) : PARHIP_ERR_NOMEM;
+	return PARHIP_OK;
+}
+
+int parhip_par2_set_current_slice_size(parhip_par2* h, size_t size) {
+	if (!h) return fail(PARHIP_ERR_ARG, "par2: null handle");
+	h->be->setCurrentSliceSize(size);
+	return be_status(h);
+}
+
+int parhip_par2_can_add(parhip_par2* h) { return h ? (int)h->be->canAdd() : PARHIP_ERR_ARG; }
+
+int parhip_par2_add_input(parhip_par2* h, const void* buffer, size_t size, uint16_t inputNum,
+                          int flush) {
+	if (!h || (!buffer && size)) return fail(PARHIP_ERR_ARG, "par2_add_input: null pointer");
+	h->be->addInput(buffer, size, inputNum, flush != 0).get();
+	return be_status(h);
+}
+
+int parhip_par2_add_input_coeffs(parhip_par2* h, const void* buffer, size_t size,
+                                 const uint16_t* coeffs, int flush) {
+	if (!h || (!buffer && size) || !coeffs)
+		return fail(PARHIP_ERR_ARG, "par2_add_input_coeffs: null pointer");
+	h->be->addInput(buffer, size, coeffs, flush != 0).get();
+	return be_status(h);
+}
+
+int parhip_par2_flush(parhip_par2* h) {
+	if (!h) return fail(PARHIP_ERR_ARG, "par2: null handle");
+	h->be->flush();
+	return be_status(h);
+}
+
+int parhip_par2_end_input(parhip_par2* h) {
+	if (!h) return fail(PARHIP_ERR_ARG, "par2: null handle");
+	h->be->endInput().get();
+	return be_status(h);
+}
+
+int parhip_par2_get_output(parhip_par2* h, unsigned index, void* output) {
+	if (!h || !output) return fail(PARHIP_ERR_ARG, "par2_get_output: null pointer");
+	if ((int)index >= h->be->getNumRecoverySlices())
+		return fail(PARHIP_ERR_ARG, "par2_get_output: index %u out of range", index);
+	bool valid = h->be->getOutput(index, output).get();
+	int rc = be_status(h);
+	return rc ? rc : (valid ? 1 : 0);
+}
+
+int parhip_par2_discard_output(parhip_par2* h) {
+	if (!h) return fail(PARHIP_ERR_ARG, "par2: null handle");
+	h->be->discardOutput();
+	return PARHIP_OK;
+}
+
+int parhip_par2_input_batch_size(parhip_par2* h) {
+	return h ? (int)h->be->getInputBatchSize() : PARHIP_ERR_ARG;
+}
+
+}  // extern "C"

@@ -69,3 +69,34 @@ def recovery_ptrs(in_ptrs, out_ptrs, words, coeff, accumulate=True, stream=None)
     check(lib().parhip_d_gf64_recovery_ptrs(in_ptrs.data_ptr(), n, out_ptrs.data_ptr(), r, words,
                                             coeff.data_ptr(), coeff.stride(0),
                                             1 if accumulate else 0, _stream(stream)))
+
+
+# ------------------------------------------------------------ PAR2 GF(2^16)
+
+def gf16_coeffs(input_nums, exponents, device="cuda", stream=None):
+    """(R, N) int16 device tensor of PAR2 coefficients (bit patterns)."""
+    import numpy as np
+    inums = np.ascontiguousarray(input_nums, dtype=np.uint16)
+    exps = np.ascontiguousarray(exponents, dtype=np.uint16)
+    out = torch.empty((exps.size, inums.size), dtype=torch.int16, device=device)
+    check(lib().parhip_d_gf16_coeffs(out.data_ptr(), inums.ctypes.data, inums.size,
+                                     exps.ctypes.data, exps.size, _stream(stream)))
+    return out
+
+
+def gf16_muladd(inputs, coeff, out=None, words=None, accumulate=False, stream=None):
+    """out[e] (^)= XOR_i coeff[e, i] * inputs[i] over 16-bit words.
+    inputs: (N, >=words) int16 device tensor; coeff: (R, N) int16 device tensor."""
+    if inputs.dtype != torch.int16 or inputs.dim() != 2 or inputs.stride(1) != 1:
+        raise TypeError("inputs must be a row-contiguous (N, words) int16 device tensor")
+    n, w = inputs.shape
+    words = w if words is None else words
+    r = coeff.shape[0]
+    if coeff.shape[1] != n or not coeff.is_contiguous() or coeff.dtype != torch.int16:
+        raise ValueError("coeff must be a contiguous (R, N) int16 tensor")
+    if out is None:
+        out = torch.empty((r, words), dtype=torch.int16, device=inputs.device)
+    check(lib().parhip_d_gf16_muladd(inputs.data_ptr(), n, inputs.stride(0), out.data_ptr(), r,
+                                     out.stride(0), words, coeff.data_ptr(), 1 if accumulate else 0,
+                                     _stream(stream)))
+    return out

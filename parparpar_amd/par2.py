@@ -1,0 +1,88 @@
+"""PAR2 recovery over the PAR2ProcHIP backend, in the call shapes of the
+reference's GfProc (src/gf.cc:239-735, used by lib/par2.js:112-228).
+
+    gp = GfProc(sliceSize)                  # new binding.GfProc(sliceSize, ...)
+    gp.set_recovery_slices([0, 1, 2, ...])  # exponents (lib/par2.js:210-228)
+    gp.add(inputNum, buf)                   # gf.add(idx, buf, cb)
+    gp.end()                                # gf.end(cb)
+    valid = gp.get(idx, out)                # gf.get(idx, buf, cb(idx, cksumValid, buf))
+
+All work runs in libparhip.so; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import ParhipError, check, lib
+
+
+class GfProc:
+    def __init__(self, slice_size, device=0, input_batch=0):
+        self._h = lib().parhip_par2_create(device, slice_size, input_batch)
+        if not self._h:
+            raise ParhipError(-4, lib().parhip_last_error().decode())
+        self.slice_size = slice_size
+        self.num_outputs = 0
+
+    def close(self):
+        if self._h:
+            lib().parhip_par2_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_recovery_slices(self, exponents):
+        """exponents: list of recovery exponents, or an int n for custom-coefficient mode."""
+        if isinstance(exponents, int):
+            check(lib().parhip_par2_set_recovery_slices(self._h, exponents, None))
+            self.num_outputs = exponents
+        else:
+            e = np.ascontiguousarray(exponents, dtype=np.uint16)
+            check(lib().parhip_par2_set_recovery_slices(self._h, e.size, e.ctypes.data))
+            self.num_outputs = e.size
+
+    def set_current_slice_size(self, size):
+        check(lib().parhip_par2_set_current_slice_size(self._h, size))
+        self.slice_size = size
+
+    def can_add(self):
+        return lib().parhip_par2_can_add(self._h)
+
+    def add(self, input_num, buf, flush=False):
+        b = np.ascontiguousarray(buf).view(np.uint8)
+        check(lib().parhip_par2_add_input(self._h, b.ctypes.data, b.size, input_num, 1 if flush else 0))
+
+    def add_coeffs(self, coeffs, buf, flush=False):
+        b = np.ascontiguousarray(buf).view(np.uint8)
+        c = np.ascontiguousarray(coeffs, dtype=np.uint16)
+        if c.size < self.num_outputs:
+            raise ValueError("need one coefficient per recovery slice")
+        check(lib().parhip_par2_add_input_coeffs(self._h, b.ctypes.data, b.size, c.ctypes.data,
+                                                 1 if flush else 0))
+
+    def flush(self):
+        check(lib().parhip_par2_flush(self._h))
+
+    def end(self):
+        check(lib().parhip_par2_end_input(self._h))
+
+    def get(self, idx, out=None):
+        """Copy recovery slice idx into `out` (uint8, slice_size); returns (cksumValid, out)."""
+        if out is None:
+            out = np.zeros(self.slice_size, dtype=np.uint8)
+        rc = lib().parhip_par2_get_output(self._h, idx, out.ctypes.data)
+        if rc < 0:
+            check(rc)
+        return rc == 1, out
+
+    def discard_output(self):
+        check(lib().parhip_par2_discard_output(self._h))
+
+    @property
+    def input_batch_size(self):
+        return lib().parhip_par2_input_batch_size(self._h)
+
+
+def gf16_coeff(input_num, exponent):
+    return lib().parhip_gf16_coeff(input_num, exponent)

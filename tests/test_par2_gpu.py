@@ -1,0 +1,136 @@
+"""PAR2 GF(2^16) parity: PAR2ProcHIP and the device muladd against the oracle.
+
+The controller cases follow the reference's own controller harness
+(test/gf16/test-ctrl.cpp:257-330): seeded random 16-bit sources, slice sizes
+{2, 19998, 20000}, last-slice sizes {1, 2, s-1, s}, inputs {1, 15, 16} x
+outputs {1, 15, 16}, random input indices, random exponents with exponent 0 in
+the first output (multi-add path); every output compared byte-for-byte and its
+checksum required valid.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def ctrl_cases():
+    cases = []
+    for s in (2, 19998, 20000):
+        lasts = [1, 2] + ([s - 1, s] if s > 2 else [])
+        for last in lasts:
+            for nout in (1, 15, 16):
+                for nin in (1, 15, 16):
+                    if nin == 1 and last != s:
+                        continue
+                    if last != s and last != 1 and (nin > 15 or nout > 2):
+                        continue
+                    cases.append((s, last, nin, nout))
+    return cases
+
+
+@pytest.mark.parametrize("s,last,nin,nout", ctrl_cases())
+def test_controller_like_test_ctrl(gpu, oracle, s, last, nin, nout):
+    from parparpar_amd.par2 import GfProc
+    rng = np.random.default_rng(0x01020304 + s * 7 + last * 3 + nin * 11 + nout)
+    srcs = [rng.integers(0, 256, size=s, dtype=np.uint8) for _ in range(nin)]
+    srcs[-1] = srcs[-1][:last].copy()
+    inums = rng.integers(0, 32768, size=nin).tolist()
+    exps = rng.integers(0, 65535, size=nout).tolist()
+    exps[0] = 0
+    gp = GfProc(s, input_batch=4)
+    gp.set_recovery_slices(exps)
+    for i in range(nin):
+        gp.add(inums[i], srcs[i])
+    gp.end()
+    ref = oracle.par2_create(srcs, inums, exps, s)
+    for e in range(nout):
+        valid, out = gp.get(e)
+        assert valid, "checksum invalid for output %d" % e
+        assert np.array_equal(out, ref[e]), "output %d differs" % e
+    gp.close()
+
+
+def test_controller_custom_coeffs_and_passes(gpu, oracle):
+    from parparpar_amd.par2 import GfProc
+    rng = np.random.default_rng(9)
+    s, nin, nout = 4096, 21, 6
+    srcs = [rng.integers(0, 256, size=s, dtype=np.uint8) for _ in range(nin)]
+    coeffs = rng.integers(0, 65536, size=(nin, nout), dtype=np.uint16)
+    gp = GfProc(s, input_batch=8)
+    gp.set_recovery_slices(nout)
+    for i in range(nin):
+        gp.add_coeffs(coeffs[i], srcs[i])
+    gp.end()
+    L = oracle.lib()
+    for e in range(nout):
+        ref = np.zeros(s + 1, dtype=np.uint8)
+        for i in range(nin):
+            L.or_gf16_muladd_region(ref.ctypes.data_as(oracle._u8p), oracle.p8(srcs[i]), s,
+                                    int(coeffs[i, e]))
+        valid, out = gp.get(e)
+        assert valid and np.array_equal(out, ref[:s])
+    # second pass after discardOutput: first batch overwrites
+    gp.discard_output()
+    gp.set_recovery_slices([0, 3])
+    gp.add(5, srcs[0])
+    gp.end()
+    ref = oracle.par2_create([srcs[0]], [5], [0, 3], s)
+    for e in range(2):
+        valid, out = gp.get(e)
+        assert valid and np.array_equal(out, ref[e])
+    gp.close()
+
+
+def test_exponent0_is_xor_parity(gpu):
+    from parparpar_amd.par2 import GfProc
+    rng = np.random.default_rng(10)
+    s = 1000
+    srcs = [rng.integers(0, 256, size=s, dtype=np.uint8) for _ in range(9)]
+    gp = GfProc(s)
+    gp.set_recovery_slices([0])
+    for i, b in enumerate(srcs):
+        gp.add(100 + i, b)
+    gp.end()
+    x = np.zeros(s, dtype=np.uint8)
+    for b in srcs:
+        x ^= b
+    valid, out = gp.get(0)
+    assert valid and np.array_equal(out, x)
+
+
+def test_device_coeffs(gpu, oracle):
+    import torch
+    from parparpar_amd import device
+    inums = np.arange(0, 32768, 97, dtype=np.uint16)
+    exps = np.array([0, 1, 2, 255, 511, 4097, 65534], dtype=np.uint16)
+    c = device.gf16_coeffs(inums, exps).cpu().numpy().view(np.uint16)
+    L = oracle.lib()
+    for e in range(exps.size):
+        for i in range(inums.size):
+            assert c[e, i] == L.or_gf16_coeff(int(inums[i]), int(exps[e]))
+
+
+@pytest.mark.parametrize("n,words,r", [(32768, 16384, 512), (300, 5000, 37)])
+def test_device_muladd_columns(gpu, oracle, n, words, r):
+    """1 GiB as 32768 x 32 KiB at 512 recovery slices (exponents 0..511), and a
+    ragged small shape: exact check on sampled word columns."""
+    import torch
+    from parparpar_amd import device
+    print("gf16 muladd N=%d words=%d R=%d" % (n, words, r), flush=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(n + r)
+    x = torch.randint(-32768, 32767, (n, words), dtype=torch.int16, device="cuda", generator=g)
+    inums = np.arange(n, dtype=np.uint16)
+    exps = np.arange(r, dtype=np.uint16)
+    co = device.gf16_coeffs(inums, exps)
+    out = device.gf16_muladd(x, co)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(n)
+    cols = np.unique(np.concatenate([[0, words - 1], rng.integers(0, words, size=6)])).astype(np.int64)
+    tc = torch.as_tensor(cols, device="cuda")
+    samp = np.ascontiguousarray(x[:, tc].cpu().numpy().view(np.uint8))  # (n, 2*len(cols)) bytes
+    ref = oracle.par2_columns(samp, 2 * len(cols), inums, exps, np.arange(len(cols), dtype=np.uint64))
+    got = out[:, tc].cpu().numpy().view(np.uint16)
+    assert np.array_equal(got, ref)
+    del x, out
+    torch.cuda.empty_cache()
