@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--recovery", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--workload", default="par3-create",
+                    choices=["par3-create", "par2-create", "batch", "repair"],
+                    help="par3-create (default line); the others are reported in DESIGN.md")
+    ap.add_argument("--jobs", type=int, default=200)
     return ap.parse_args()
 
 
@@ -97,6 +101,22 @@ def cpu_baseline(args, d_inputs, budget_s):
 
 def main():
     args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every GPU box here has one GPU; "
+              "--gpus %d is not supported (the path would shard slices by word range)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "par2-create":
+        return main_par2(args)
+    if args.workload == "batch":
+        return main_batch(args)
+    if args.workload == "repair":
+        import bench_repair
+        return bench_repair.main(args)
+    return main_par3(args)
+
+
+def main_par3(args):
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU; "
               "--gpus %d is not supported (the path would shard slices by word range)" % args.gpus,
@@ -194,6 +214,148 @@ def main():
             line["cpu_baseline"] = cpu_baseline(args, d_in, args.cpu_seconds)
         except Exception as e:  # reported, never silently replaced
             line["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(line), flush=True)
+
+
+def main_par2(args):
+    """PAR2 create, 1 GiB = 32,768 slices x 32 KiB, 512 recovery slices
+    (exponents 0..511), inputs resident in HBM; one step = coefficients on the
+    device + the GF(2^16) contraction."""
+    import numpy as np
+    import torch
+    import parparpar_amd as P
+    torch.cuda.set_device(0)
+    L = P.lib()
+    n, sb, r = 32768, 32768, 512
+    words = sb // 2
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0x5EED2)
+    x = torch.randint(-32768, 32767, (n, words), dtype=torch.int16, device="cuda", generator=g)
+    out = torch.empty((r, words), dtype=torch.int16, device="cuda")
+    co = torch.empty((r, n), dtype=torch.int16, device="cuda")
+    inums = np.arange(n, dtype=np.uint16)
+    exps = np.arange(r, dtype=np.uint16)
+    sp = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        P.check(L.parhip_d_gf16_coeffs(co.data_ptr(), inums.ctypes.data, n, exps.ctypes.data, r, sp))
+        P.check(L.parhip_d_gf16_muladd(x.data_ptr(), n, words, out.data_ptr(), r, words, words,
+                                       co.data_ptr(), 0, sp))
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    P.prof_enable(True)
+    P.prof_reset()
+    steps = max(1, min(args.steps, 10))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    P.prof_enable(False)
+    tab_ms, tab_n = P.prof_get("gf16_tab")
+    red_ms, red_n = P.prof_get("gf16_reduce")
+    k_s = tab_ms / max(tab_n, 1) / 1e3
+    macs = n * r * words  # GF(2^16) multiply-accumulates per launch
+    bit_ops = 2.0 * macs * 256  # as a GF(2) bit-matrix contraction (16x16 per MAC)
+    line = {
+        "metric": METRIC, "value": round(n * sb * steps / 2 ** 30 / el, 3), "unit": "GiB/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(el / steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u16", "data": "synthetic (seeded uniform random bytes, resident in HBM)",
+        "config": {"workload": "PAR2 create 1 GiB = 32768 slices x 32 KiB, 512 recovery slices (exponents 0..511)",
+                   "method": "GF(2^16) nibble split tables in LDS (k_gf16_tab) + split-K XOR reduce"},
+        "kernels_ms_avg": {"gf16_tab": round(tab_ms / max(tab_n, 1), 3),
+                           "gf16_reduce": round(red_ms / max(red_n, 1), 3)},
+        "roofline": {"bound": "mfma", "achieved": round(bit_ops / k_s / 1e12, 1), "peak": 5000.0,
+                     "unit": "TFLOP/s", "frac": round(bit_ops / k_s / 1e12 / 5000.0, 4), "traffic": None,
+                     "note": "GF(2) bit-operation equivalent against the int8 MFMA peak (5.0 POP/s)"},
+        "hbm_GBps": round((n * sb + r * sb) / k_s / 1e9, 1),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def make_uneven_jobs(njobs, seed=0xC4):
+    """200 jobs: size ~ LogUniform[4 MiB, 512 MiB], slice in {4, 8, 32, 1024} KiB,
+    R ~ U{4..64}; last slice zero-padded. Returns (sizes, slice_bytes, R) lists."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(4 << 20), np.log(512 << 20)
+    sizes = (np.exp(rng.uniform(lo, hi, size=njobs)).astype(np.int64) // 8 * 8).tolist()
+    slices = rng.choice([4 << 10, 8 << 10, 32 << 10, 1 << 20], size=njobs).tolist()
+    recs = rng.integers(4, 65, size=njobs).tolist()
+    return sizes, slices, recs
+
+
+def main_batch(args):
+    """Batched uneven PAR3 create: persistent kernel + device queue + hipGraph
+    (autotuned launch variants) vs plain per-job launches, aggregate GiB/s."""
+    import torch
+    import parparpar_amd as P
+    from parparpar_amd.batch import Batch
+    torch.cuda.set_device(0)
+    L = P.lib()
+    sizes, slices, recs = make_uneven_jobs(args.jobs)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0xC4)
+    jobs = []
+    for size, sb, r in zip(sizes, slices, recs):
+        n = (size + sb - 1) // sb
+        w = sb // 8
+        x = torch.randint(-2 ** 63, 2 ** 63 - 1, (n, w), dtype=torch.int64, device="cuda", generator=g)
+        tail = n * w - size // 8
+        if tail:
+            x.view(-1)[size // 8:] = 0  # zero-padded last slice
+        y = torch.empty((r, w), dtype=torch.int64, device="cuda")
+        jobs.append((x, y, 0, n))
+    total = sum(sizes)
+    sp = torch.cuda.current_stream().cuda_stream
+    co = [torch.empty((y.shape[0], x.shape[0]), dtype=torch.int64, device="cuda") for (x, y, _, _) in jobs]
+
+    def plain():
+        for k, (x, y, fi, fr) in enumerate(jobs):
+            n, w = x.shape
+            r = y.shape[0]
+            P.check(L.parhip_d_gf64_build_cauchy(co[k].data_ptr(), n, r, fi, fr, sp))
+            P.check(L.parhip_d_gf64_recovery_with_coeff(x.data_ptr(), n, w, y.data_ptr(), r, w, w,
+                                                        co[k].data_ptr(), 0, sp))
+    t0 = time.perf_counter()
+    b = Batch(jobs, graph=True, autotune=True)
+    plan_s = time.perf_counter() - t0
+    steps = max(1, min(args.steps, 5))
+    res = {}
+    for name, fn in (("plain", plain), ("batched", b.run)):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / steps
+    # bit-exact: batched outputs equal the plain outputs
+    outs = [y.clone() for (_, y, _, _) in jobs]
+    plain()
+    torch.cuda.synchronize()
+    exact = all(torch.equal(o, y) for o, (_, y, _, _) in zip(outs, jobs))
+    gib = total / 2 ** 30
+    line = {
+        "metric": METRIC, "value": round(gib / res["batched"], 3), "unit": "GiB/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(res["batched"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded uniform random bytes, resident in HBM)",
+        "config": {"workload": "batched uneven PAR3 create: %d jobs, %.2f GiB total, size LogUniform[4 MiB, 512 MiB], "
+                               "slice {4,8,32,1024} KiB, R U{4..64}" % (len(jobs), gib),
+                   "method": "persistent k_batch_persistent + device tile queue + hipGraph, autotuned variants"},
+        "plain_GiBps": round(gib / res["plain"], 3),
+        "batched_GiBps": round(gib / res["batched"], 3),
+        "batched_vs_plain": round(res["plain"] / res["batched"], 3),
+        "plan_s": round(plan_s, 3),
+        "batch_info": b.info(),
+        "bit_exact_batched_vs_plain": bool(exact),
+        "cpu_baseline": None,
+    }
     print(json.dumps(line), flush=True)
 
 

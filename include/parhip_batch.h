@@ -1,0 +1,62 @@
+/*
+ * parhip_batch.h -- batched PAR3 create and the launch-shape autotuner.
+ *
+ * No single counterpart in the reference, which runs one create per call
+ * (lib/par3gen.js:2441 -> ComputeRecoveryBlocksWithCoeff, src/par3_engine.cc:1165)
+ * and only times block sizes for telemetry (PAR3_GF64_BLOCK_SIZE_AUTOTUNE,
+ * src/par3_engine.cc:655-732).  Each job below has exactly the value
+ * semantics of parhip_d_gf64_recovery (include/parhip.h) with accumulate = 0:
+ * recovery[r] = XOR_i cauchy(firstInput+i, firstRecovery+r) * input[i].
+ */
+#ifndef PARHIP_BATCH_H
+#define PARHIP_BATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+	const uint64_t *d_inputs; /* device: numInputs slices, inputStride64 words apart */
+	size_t numInputs;
+	size_t inputStride64;
+	uint64_t *d_recovery;     /* device: numRecovery slices, recoveryStride64 words apart */
+	size_t numRecovery;
+	size_t recoveryStride64;
+	size_t blockSize64;       /* words per slice */
+	uint64_t firstInput;
+	uint64_t firstRecovery;
+} parhip_gf64_job;
+
+typedef struct parhip_batch parhip_batch;
+
+#define PARHIP_BATCH_GRAPH 1u    /* capture the launch sequence in a hipGraph */
+#define PARHIP_BATCH_AUTOTUNE 2u /* time the launch variants per job shape (cached) */
+
+/* Plan a batch: per-job launch variant (autotuned when asked), the device tile
+ * queue, coefficient storage; optionally capture the graph. Buffers named by
+ * the jobs must stay valid while the batch is used. NULL on failure. */
+parhip_batch *parhip_batch_create(const parhip_gf64_job *jobs, size_t njobs, unsigned flags);
+/* Enqueue one run of every job on `stream` (graph replay when captured). */
+int parhip_batch_run(parhip_batch *b, void *stream);
+int parhip_batch_info(const parhip_batch *b, uint64_t *ntiles, uint64_t *natomic_jobs,
+                      uint64_t *coeff_bytes);
+/* Launch variant chosen for job `job`: outputs per work-group, words per
+ * thread, inputs per LDS batch. */
+int parhip_batch_job_config(const parhip_batch *b, size_t job, int *rt, int *wpt, int *ib);
+void parhip_batch_destroy(parhip_batch *b);
+
+/* Autotuner cache: look up a tuned shape, persist, reload, clear. */
+int parhip_autotune_lookup(size_t numInputs, size_t numRecovery, size_t blockSize64, int *rt,
+                           int *wpt, int *ib, double *ns_per_mac);
+int parhip_autotune_save(const char *path);
+int parhip_autotune_load(const char *path);
+int parhip_autotune_clear(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARHIP_BATCH_H */

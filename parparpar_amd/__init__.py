@@ -68,6 +68,25 @@ _SIGS = [
     ("parhip_par2_get_output", _int, [_vp, ctypes.c_uint, _vp]),
     ("parhip_par2_discard_output", _int, [_vp]),
     ("parhip_par2_input_batch_size", _int, [_vp]),
+    ("parhip_batch_create", _vp, [_vp, _sz, ctypes.c_uint]),
+    ("parhip_batch_run", _int, [_vp, _vp]),
+    ("parhip_batch_info", _int, [_vp, _vp, _vp, _vp]),
+    ("parhip_batch_job_config", _int, [_vp, _sz, _vp, _vp, _vp]),
+    ("parhip_batch_destroy", None, [_vp]),
+    ("parhip_autotune_lookup", _int, [_sz, _sz, _sz, _vp, _vp, _vp, _vp]),
+    ("parhip_autotune_save", _int, [ctypes.c_char_p]),
+    ("parhip_autotune_load", _int, [ctypes.c_char_p]),
+    ("parhip_autotune_clear", _int, []),
+    ("parhip_d_gf64_repair", _int, [_vp, _sz, _sz, _vp, _sz, _sz, _sz, _u64, _u64, _vp, _sz, _vp, _vp, _sz, _vp]),
+    ("parhip_gf64_repair_workspace_bytes", _sz, [_sz, _sz, _sz]),
+    ("parhip_repair_server_create", _vp, [_sz, ctypes.c_uint, ctypes.c_uint]),
+    ("parhip_repair_server_add_archive", _int, [_vp, _vp, _sz, _sz, _vp, _sz, _sz, _sz, _u64, _u64]),
+    ("parhip_repair_submit", ctypes.c_int64, [_vp, _int, _vp, _sz]),
+    ("parhip_repair_wait", _int, [_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]),
+    ("parhip_repair_result", _int, [_vp, ctypes.c_int64, ctypes.POINTER(_vp), ctypes.POINTER(_sz)]),
+    ("parhip_repair_release", _int, [_vp, ctypes.c_int64]),
+    ("parhip_repair_server_stats", _int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("parhip_repair_server_destroy", None, [_vp]),
     ("parhip_prof_enable", _int, [_int]),
     ("parhip_prof_reset", _int, []),
     ("parhip_prof_get", _int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),

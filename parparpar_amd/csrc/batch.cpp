@@ -1,0 +1,446 @@
+// batch.cpp -- batched PAR3 create: many independent jobs of uneven input
+// count, slice size and recovery count in one persistent kernel launch.
+//
+// The reference runs each create as its own call (lib/par3gen.js:2441 ->
+// ComputeRecoveryBlocksWithCoeff, src/par3_engine.cc:1165); here a batch of
+// jobs is planned once into a device queue of tiles (job, word range, output
+// range, input range), each tile using the launch variant the autotuner chose
+// for its job's shape, and the whole sequence -- every job's Cauchy matrix,
+// zeroing of split jobs' outputs, queue reset, persistent kernel -- is
+// captured in a hipGraph and replayed.  Jobs whose input range is split over
+// several tiles accumulate with 64-bit atomic XOR (order-independent: the
+// bytes are identical to the unsplit computation).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/parhip.h"
+#include "../../include/parhip_batch.h"
+#include "engine_internal.h"
+#include "kernels.h"
+
+namespace parhip {
+
+struct CfgInfo {
+	int rt, wpt, ib;
+};
+static const CfgInfo kCfg[BATCH_CFG_COUNT] = {
+	{8, 4, 4}, {8, 2, 4}, {0, 0, 0} /* unused */, {16, 2, 2}, {4, 4, 4}};
+
+static std::vector<int> candidates(uint64_t R, uint64_t W) {
+	std::vector<int> c;
+	auto add = [&](int id) {
+		if (kCfg[id].wpt == 4 && W <= 512) return;  // a 1024-word tile would idle half the threads
+		c.push_back(id);
+	};
+	if (R <= 4) add(BATCH_CFG_4_4_4);
+	add(BATCH_CFG_8_4_4);
+	add(BATCH_CFG_8_2_4);
+	if (R > 8) add(BATCH_CFG_16_2_2);
+	if (c.empty()) c.push_back(BATCH_CFG_8_2_4);
+	return c;
+}
+
+// ---------------------------------------------------------------- tile planning
+static const uint64_t kTileWork = 1ull << 22;  // word x output MACs per tile
+
+static uint32_t plan_job(uint32_t jid, const BatchJob& jb, int cfg, std::vector<BatchTile>& out,
+                         bool* atomic) {
+	const CfgInfo& c = kCfg[cfg];
+	const uint64_t tw = 256ull * c.wpt;
+	const uint64_t wt = (jb.W + tw - 1) / tw;
+	const uint64_t rtn = (jb.R + c.rt - 1) / c.rt;
+	uint64_t chunk = kTileWork / (tw * c.rt);
+	chunk = std::max<uint64_t>(chunk, 2ull * c.ib);
+	chunk = (chunk + c.ib - 1) / c.ib * c.ib;
+	if (chunk >= jb.N) chunk = jb.N;
+	const uint64_t nch = (jb.N + chunk - 1) / chunk;
+	*atomic = nch > 1;
+	uint32_t n = 0;
+	for (uint64_t k = 0; k < nch; k++)
+		for (uint64_t rt = 0; rt < rtn; rt++)
+			for (uint64_t w = 0; w < wt; w++) {
+				BatchTile t;
+				t.job = jid;
+				t.cfg = (uint16_t)cfg;
+				t.mode = *atomic ? 2 : 0;
+				t.r0 = rt * c.rt;
+				t.w0 = w * tw;
+				t.i0 = k * chunk;
+				t.i1 = std::min<uint64_t>(jb.N, t.i0 + chunk);
+				out.push_back(t);
+				n++;
+			}
+	return n;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Per-shape choice of the launch variant, measured on the device with the
+// job's own buffers and cached by (words per slice, R, input-count bucket).
+struct TuneKey {
+	uint64_t W, R, nb;
+	bool operator<(const TuneKey& o) const { return std::tie(W, R, nb) < std::tie(o.W, o.R, o.nb); }
+};
+struct TuneVal {
+	int cfg;
+	double ns_per_mac;
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneVal> g_tune;
+
+static uint64_t n_bucket(uint64_t N) {
+	uint64_t b = 1;
+	while (b < N && b < (1ull << 20)) b <<= 1;
+	return b;
+}
+
+static int run_single(const BatchJob& jb, int cfg, BatchJob* djob, BatchTile* dtiles,
+                      uint32_t* dcounter, uint32_t* dzero, int blocks, hipStream_t s,
+                      float* ms_out) {
+	std::vector<BatchTile> tiles;
+	bool atomic = false;
+	BatchJob j = jb;
+	plan_job(0, j, cfg, tiles, &atomic);
+	hipMemcpyAsync(djob, &j, sizeof(BatchJob), hipMemcpyHostToDevice, s);
+	hipMemcpyAsync(dtiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice, s);
+	uint32_t zid = 0;
+	hipMemcpyAsync(dzero, &zid, sizeof(uint32_t), hipMemcpyHostToDevice, s);
+	hipEvent_t a, b;
+	hipEventCreate(&a);
+	hipEventCreate(&b);
+	float best = 1e30f;
+	for (int rep = 0; rep < 3; rep++) {
+		hipEventRecord(a, s);
+		hipError_t e = launch_batch(djob, 1, dtiles, (uint32_t)tiles.size(), dcounter, dzero,
+		                            atomic ? 1 : 0, 0, blocks, s);
+		hipEventRecord(b, s);
+		if (e != hipSuccess) {
+			hipEventDestroy(a);
+			hipEventDestroy(b);
+			return hip_fail(e, "autotune launch");
+		}
+		hipEventSynchronize(b);
+		float ms = 0;
+		hipEventElapsedTime(&ms, a, b);
+		if (rep && ms < best) best = ms;  // first run warms up
+	}
+	hipEventDestroy(a);
+	hipEventDestroy(b);
+	*ms_out = best;
+	return PARHIP_OK;
+}
+
+// Tune (or look up) the variant for a job; `jb.coeff` must hold its matrix.
+static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_out) {
+	TuneKey key{jb.W, jb.R, n_bucket(jb.N)};
+	{
+		std::lock_guard<std::mutex> lk(g_tune_mu);
+		auto it = g_tune.find(key);
+		if (it != g_tune.end()) {
+			*cfg_out = it->second.cfg;
+			return PARHIP_OK;
+		}
+	}
+	std::vector<int> cand = candidates(jb.R, jb.W);
+	if (cand.size() == 1) {
+		*cfg_out = cand[0];
+		std::lock_guard<std::mutex> lk(g_tune_mu);
+		g_tune[key] = {cand[0], 0.0};
+		return PARHIP_OK;
+	}
+	// scratch output so the caller's recovery buffer is not disturbed
+	DevBuf dout, djob, dtiles, dcnt, dzero;
+	BatchJob j = jb;
+	int rc;
+	if ((rc = dout.alloc(jb.R * jb.W * sizeof(uint64_t), s)) ||
+	    (rc = djob.alloc(sizeof(BatchJob), s)) || (rc = dcnt.alloc(sizeof(uint32_t), s)) ||
+	    (rc = dzero.alloc(sizeof(uint32_t), s)))
+		return rc;
+	j.out = dout.as<uint64_t>();
+	j.out_stride = jb.W;
+	size_t max_tiles = 0;
+	for (int c : cand) {
+		std::vector<BatchTile> t;
+		bool at;
+		max_tiles = std::max<size_t>(max_tiles, plan_job(0, j, c, t, &at));
+	}
+	if ((rc = dtiles.alloc(max_tiles * sizeof(BatchTile), s))) return rc;
+	int best = cand[0];
+	double best_ns = 1e300;
+	const double macs = (double)jb.N * jb.W * jb.R;
+	for (int c : cand) {
+		float ms = 0;
+		rc = run_single(j, c, djob.as<BatchJob>(), dtiles.as<BatchTile>(), dcnt.as<uint32_t>(),
+		                dzero.as<uint32_t>(), blocks, s, &ms);
+		if (rc) return rc;
+		const double ns = ms * 1e6 / macs;
+		if (ns < best_ns) {
+			best_ns = ns;
+			best = c;
+		}
+	}
+	hipStreamSynchronize(s);
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	g_tune[key] = {best, best_ns};
+	*cfg_out = best;
+	return PARHIP_OK;
+}
+
+}  // namespace parhip
+
+using namespace parhip;
+
+// ====================================================================== batch object
+struct parhip_batch {
+	int dev = 0;
+	std::vector<BatchJob> jobs;
+	std::vector<int> cfg;
+	BatchJob* djobs = nullptr;
+	BatchTile* dtiles = nullptr;
+	uint32_t ntiles = 0;
+	uint32_t* dcounter = nullptr;
+	uint32_t* dzero = nullptr;
+	uint32_t nzero = 0;
+	uint64_t* dcoeff = nullptr;
+	uint64_t coeff_total = 0;
+	int blocks = 0;
+	hipStream_t cap = nullptr;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	uint64_t tile_counts[BATCH_CFG_COUNT] = {0};
+};
+
+static void batch_free(parhip_batch* b) {
+	if (!b) return;
+	hipSetDevice(b->dev);
+	if (b->cap) hipStreamSynchronize(b->cap);
+	if (b->exec) hipGraphExecDestroy(b->exec);
+	if (b->graph) hipGraphDestroy(b->graph);
+	if (b->djobs) hipFree(b->djobs);
+	if (b->dtiles) hipFree(b->dtiles);
+	if (b->dcounter) hipFree(b->dcounter);
+	if (b->dzero) hipFree(b->dzero);
+	if (b->dcoeff) hipFree(b->dcoeff);
+	if (b->cap) hipStreamDestroy(b->cap);
+	delete b;
+}
+
+extern "C" {
+
+parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, unsigned flags) {
+	if (!jobs || njobs == 0 || njobs > (1u << 30)) {
+		fail(PARHIP_ERR_ARG, "batch_create: no jobs");
+		return nullptr;
+	}
+	auto* b = new (std::nothrow) parhip_batch;
+	if (!b) return nullptr;
+	b->dev = current_device();
+	hipSetDevice(b->dev);
+	const int cus = device_cus(b->dev);
+	b->blocks = cus * 2;
+	hipError_t e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+	if (e != hipSuccess) {
+		hip_fail(e, "batch stream");
+		batch_free(b);
+		return nullptr;
+	}
+	// ---- job descriptors + coefficient storage
+	for (size_t j = 0; j < njobs; j++) {
+		const parhip_gf64_job& in = jobs[j];
+		if (in.numInputs == 0 || in.numRecovery == 0 || in.blockSize64 == 0) continue;
+		if (!in.d_inputs || !in.d_recovery || in.inputStride64 < in.blockSize64 ||
+		    in.recoveryStride64 < in.blockSize64) {
+			fail(PARHIP_ERR_ARG, "batch_create: bad job %zu", j);
+			batch_free(b);
+			return nullptr;
+		}
+		BatchJob jb{};
+		jb.in = in.d_inputs;
+		jb.in_stride = in.inputStride64;
+		jb.out = in.d_recovery;
+		jb.out_stride = in.recoveryStride64;
+		jb.N = in.numInputs;
+		jb.R = in.numRecovery;
+		jb.W = in.blockSize64;
+		jb.fi = in.firstInput;
+		jb.fr = in.firstRecovery;
+		jb.coeff_off = b->coeff_total;
+		b->coeff_total += jb.N * jb.R;
+		b->jobs.push_back(jb);
+	}
+	if (b->jobs.empty()) return b;  // nothing to do: run() is a no-op
+	e = hipMalloc((void**)&b->dcoeff, b->coeff_total * sizeof(uint64_t));
+	if (e == hipSuccess) e = hipMalloc((void**)&b->djobs, b->jobs.size() * sizeof(BatchJob));
+	if (e == hipSuccess) e = hipMalloc((void**)&b->dcounter, sizeof(uint32_t));
+	if (e != hipSuccess) {
+		hip_fail(e, "batch allocation");
+		batch_free(b);
+		return nullptr;
+	}
+	for (auto& jb : b->jobs) jb.coeff = b->dcoeff + jb.coeff_off;
+	hipMemcpy(b->djobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice);
+	// coefficients first: the autotuner runs the real kernel on them
+	e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), nullptr, 0, b->dcounter, nullptr, 0,
+	                 b->coeff_total, 0, b->cap);
+	hipStreamSynchronize(b->cap);
+	// ---- per-job launch variant
+	b->cfg.resize(b->jobs.size());
+	for (size_t j = 0; j < b->jobs.size(); j++) {
+		const BatchJob& jb = b->jobs[j];
+		int cfg = jb.R <= 4 ? BATCH_CFG_4_4_4 : (jb.W <= 512 ? BATCH_CFG_8_2_4 : BATCH_CFG_8_4_4);
+		if (flags & PARHIP_BATCH_AUTOTUNE) {
+			if (autotune_job(jb, b->blocks, b->cap, &cfg)) {
+				batch_free(b);
+				return nullptr;
+			}
+		}
+		b->cfg[j] = cfg;
+	}
+	// ---- tiles: interleave jobs so concurrent work-groups mix shapes
+	std::vector<std::vector<BatchTile>> per(b->jobs.size());
+	std::vector<uint32_t> zero_ids;
+	for (size_t j = 0; j < b->jobs.size(); j++) {
+		bool atomic = false;
+		plan_job((uint32_t)j, b->jobs[j], b->cfg[j], per[j], &atomic);
+		b->tile_counts[b->cfg[j]] += per[j].size();
+		if (atomic) zero_ids.push_back((uint32_t)j);
+	}
+	std::vector<BatchTile> tiles;
+	size_t total = 0;
+	for (auto& v : per) total += v.size();
+	if (total > 0xFFFFFFF0ull) {
+		fail(PARHIP_ERR_RANGE, "batch_create: too many tiles");
+		batch_free(b);
+		return nullptr;
+	}
+	tiles.reserve(total);
+	std::vector<size_t> pos(per.size(), 0);
+	while (tiles.size() < total)
+		for (size_t j = 0; j < per.size(); j++)
+			if (pos[j] < per[j].size()) tiles.push_back(per[j][pos[j]++]);
+	b->ntiles = (uint32_t)tiles.size();
+	b->nzero = (uint32_t)zero_ids.size();
+	e = hipMalloc((void**)&b->dtiles, tiles.size() * sizeof(BatchTile));
+	if (e == hipSuccess && b->nzero) e = hipMalloc((void**)&b->dzero, b->nzero * sizeof(uint32_t));
+	if (e != hipSuccess) {
+		hip_fail(e, "batch allocation");
+		batch_free(b);
+		return nullptr;
+	}
+	hipMemcpy(b->dtiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice);
+	if (b->nzero)
+		hipMemcpy(b->dzero, zero_ids.data(), zero_ids.size() * sizeof(uint32_t),
+		          hipMemcpyHostToDevice);
+	// ---- graph: Cauchy build, zeroing, queue reset, persistent kernel
+	if (flags & PARHIP_BATCH_GRAPH) {
+		e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
+		if (e == hipSuccess)
+			e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), b->dtiles, b->ntiles, b->dcounter,
+			                 b->dzero, b->nzero, b->coeff_total, b->blocks, b->cap);
+		hipError_t e2 = hipStreamEndCapture(b->cap, &b->graph);
+		if (e == hipSuccess) e = e2;
+		if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+		if (e != hipSuccess) {
+			hip_fail(e, "batch graph capture");
+			batch_free(b);
+			return nullptr;
+		}
+	}
+	return b;
+}
+
+int parhip_batch_run(parhip_batch* b, void* stream) {
+	if (!b) return fail(PARHIP_ERR_ARG, "batch_run: null batch");
+	if (b->jobs.empty()) return PARHIP_OK;
+	hipStream_t s = (hipStream_t)stream;
+	hipError_t e;
+	ProfScope ps("batch", s);
+	if (b->exec) {
+		e = hipGraphLaunch(b->exec, s);
+	} else {
+		e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), b->dtiles, b->ntiles, b->dcounter,
+		                 b->dzero, b->nzero, b->coeff_total, b->blocks, s);
+	}
+	return e == hipSuccess ? PARHIP_OK : hip_fail(e, "batch_run");
+}
+
+int parhip_batch_info(const parhip_batch* b, uint64_t* ntiles, uint64_t* natomic_jobs,
+                      uint64_t* coeff_bytes) {
+	if (!b) return fail(PARHIP_ERR_ARG, "batch_info: null batch");
+	if (ntiles) *ntiles = b->ntiles;
+	if (natomic_jobs) *natomic_jobs = b->nzero;
+	if (coeff_bytes) *coeff_bytes = b->coeff_total * sizeof(uint64_t);
+	return PARHIP_OK;
+}
+
+int parhip_batch_job_config(const parhip_batch* b, size_t job, int* rt, int* wpt, int* ib) {
+	if (!b || job >= b->cfg.size()) return fail(PARHIP_ERR_ARG, "batch_job_config: bad job");
+	const CfgInfo& c = kCfg[b->cfg[job]];
+	if (rt) *rt = c.rt;
+	if (wpt) *wpt = c.wpt;
+	if (ib) *ib = c.ib;
+	return PARHIP_OK;
+}
+
+void parhip_batch_destroy(parhip_batch* b) { batch_free(b); }
+
+int parhip_autotune_lookup(size_t numInputs, size_t numRecovery, size_t blockSize64, int* rt,
+                           int* wpt, int* ib, double* ns_per_mac) {
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	auto it = g_tune.find(TuneKey{blockSize64, numRecovery, n_bucket(numInputs)});
+	if (it == g_tune.end()) return fail(PARHIP_ERR_ARG, "autotune: shape not tuned");
+	const CfgInfo& c = kCfg[it->second.cfg];
+	if (rt) *rt = c.rt;
+	if (wpt) *wpt = c.wpt;
+	if (ib) *ib = c.ib;
+	if (ns_per_mac) *ns_per_mac = it->second.ns_per_mac;
+	return PARHIP_OK;
+}
+
+int parhip_autotune_save(const char* path) {
+	if (!path) return fail(PARHIP_ERR_ARG, "autotune_save: null path");
+	FILE* f = fopen(path, "w");
+	if (!f) return fail(PARHIP_ERR_ARG, "autotune_save: cannot open %s", path);
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	fprintf(f, "# words_per_slice recovery input_bucket cfg ns_per_mac\n");
+	for (auto& kv : g_tune)
+		fprintf(f, "%llu %llu %llu %d %.6g\n", (unsigned long long)kv.first.W,
+		        (unsigned long long)kv.first.R, (unsigned long long)kv.first.nb, kv.second.cfg,
+		        kv.second.ns_per_mac);
+	fclose(f);
+	return PARHIP_OK;
+}
+
+int parhip_autotune_load(const char* path) {
+	if (!path) return fail(PARHIP_ERR_ARG, "autotune_load: null path");
+	FILE* f = fopen(path, "r");
+	if (!f) return fail(PARHIP_ERR_ARG, "autotune_load: cannot open %s", path);
+	char line[256];
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	while (fgets(line, sizeof line, f)) {
+		if (line[0] == '#') continue;
+		unsigned long long W, R, nb;
+		int cfg;
+		double ns;
+		if (sscanf(line, "%llu %llu %llu %d %lg", &W, &R, &nb, &cfg, &ns) == 5 && cfg >= 0 &&
+		    cfg < BATCH_CFG_COUNT && kCfg[cfg].rt)
+			g_tune[TuneKey{W, R, nb}] = {cfg, ns};
+	}
+	fclose(f);
+	return PARHIP_OK;
+}
+
+int parhip_autotune_clear(void) {
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	g_tune.clear();
+	return PARHIP_OK;
+}
+
+}  // extern "C"

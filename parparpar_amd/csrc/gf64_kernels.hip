@@ -78,22 +78,25 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 	return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
+// One tile of the contraction: words [w0, w0 + 256*WPT), outputs [r0, r0+RT),
+// inputs [i_begin, i_end).  `tab` is IB*(RT/2)*256 uint4 of LDS.  out_mode:
+//   OUT_DIRECT  store (or XOR when a.accumulate) into the outputs,
+//   OUT_PARTIAL store into a.partial row g (split-K, reduced afterwards),
+//   OUT_ATOMIC  64-bit atomic XOR into the outputs (split-K inside one
+//               persistent launch; XOR is order-independent, so the result is
+//               deterministic).
+enum { OUT_DIRECT = 0, OUT_PARTIAL = 1, OUT_ATOMIC = 2 };
+
 template <int RT, int WPT, int IB>
-__global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
+__device__ __forceinline__ void mac_tab_tile(const MacArgs& a, uint4* tab, uint64_t w0,
+                                             uint64_t r0, uint64_t i_begin, uint64_t i_end,
+                                             uint64_t g, int out_mode) {
 	static_assert(RT % 2 == 0 && RT >= 2 && RT <= 16, "RT");
 	static_assert(WPT % 2 == 0 && WPT >= 2 && WPT <= 8, "WPT");
-	constexpr int NP = RT / 2;               // output pairs
-	constexpr int TILE = 256 * WPT;          // words per work-group
-	// tab[ib][p][k][n] : 16 bytes = (c_{2p} * n x^{4k}, c_{2p+1} * n x^{4k})
-	__shared__ uint4 tab[IB][NP][16][16];
-
+	constexpr int NP = RT / 2;  // output pairs
+	// tab[((ib*NP + p)*16 + k)*16 + n] = (c_{2p} * n x^{4k}, c_{2p+1} * n x^{4k})
+#define TAB(ib, p, k, n) tab[(((ib) * NP + (p)) * 16 + (k)) * 16 + (n)]
 	const int tid = threadIdx.x;
-	const uint64_t w0 = (uint64_t)blockIdx.x * TILE;
-	const uint64_t r0 = (uint64_t)blockIdx.z * RT;
-	const uint64_t g = blockIdx.y;
-	const uint64_t i_begin = g * a.in_per_group;
-	uint64_t i_end = i_begin + a.in_per_group;
-	if (i_end > a.N) i_end = a.N;
 
 	// word positions owned by this thread: pairs (2*tid + 512*j, +1);
 	// acc[j][p] = (lo, hi) of output 2p and (lo, hi) of output 2p+1
@@ -164,7 +167,7 @@ __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 				v.y = (uint32_t)(e0[n] >> 32);
 				v.z = (uint32_t)e1[n];
 				v.w = (uint32_t)(e1[n] >> 32);
-				tab[ib][p][k][n] = v;
+				TAB(ib, p, k, n) = v;
 			}
 		}
 		__syncthreads();
@@ -185,10 +188,10 @@ __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 					}
 #pragma unroll
 					for (int p = 0; p < NP; p++) {
-						const uint4 v0 = tab[ib][p][kq + 0][n[0]];
-						const uint4 v1 = tab[ib][p][kq + 1][n[1]];
-						const uint4 v2 = tab[ib][p][kq + 2][n[2]];
-						const uint4 v3 = tab[ib][p][kq + 3][n[3]];
+						const uint4 v0 = TAB(ib, p, kq + 0, n[0]);
+						const uint4 v1 = TAB(ib, p, kq + 1, n[1]);
+						const uint4 v2 = TAB(ib, p, kq + 2, n[2]);
+						const uint4 v3 = TAB(ib, p, kq + 3, n[3]);
 						uint4 c = acc[j][p];
 						c.x = xor3(xor3(c.x, v0.x, v1.x), v2.x, v3.x);
 						c.y = xor3(xor3(c.y, v0.y, v1.y), v2.y, v3.y);
@@ -200,15 +203,16 @@ __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 			}
 		}
 	}
+#undef TAB
 
-	// ---- write out: partial buffer (split-K) or the outputs themselves
+	// ---- write out
 #pragma unroll
 	for (int r = 0; r < RT; r++) {
 		const uint64_t rr = r0 + r;
 		if (rr >= a.R) break;
 		uint64_t* dst;
 		bool xr;
-		if (a.partial) {
+		if (out_mode == OUT_PARTIAL) {
 			dst = a.partial + (g * a.R + rr) * a.W;
 			xr = false;
 		} else {
@@ -222,7 +226,11 @@ __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 			const uint4 c0 = acc[2 * j][p], c1 = acc[2 * j + 1][p];
 			uint64_t o0 = (r & 1) ? (((uint64_t)c0.w << 32) | c0.z) : (((uint64_t)c0.y << 32) | c0.x);
 			uint64_t o1 = (r & 1) ? (((uint64_t)c1.w << 32) | c1.z) : (((uint64_t)c1.y << 32) | c1.x);
-			if (w + 1 < a.W && ((reinterpret_cast<uintptr_t>(dst) & 15u) == 0)) {
+			if (out_mode == OUT_ATOMIC) {
+				if (w < a.W && o0) atomicXor((unsigned long long*)(dst + w), (unsigned long long)o0);
+				if (w + 1 < a.W && o1)
+					atomicXor((unsigned long long*)(dst + w + 1), (unsigned long long)o1);
+			} else if (w + 1 < a.W && ((reinterpret_cast<uintptr_t>(dst) & 15u) == 0)) {
 				ulonglong2* q = reinterpret_cast<ulonglong2*>(dst + w);
 				if (xr) {
 					const ulonglong2 o = *q;
@@ -237,6 +245,100 @@ __global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
 				if (w < a.W) dst[w] = xr ? (dst[w] ^ o0) : o0;
 				if (w + 1 < a.W) dst[w + 1] = xr ? (dst[w + 1] ^ o1) : o1;
 			}
+		}
+	}
+}
+
+template <int RT, int WPT, int IB>
+__global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
+	__shared__ uint4 tab[IB * (RT / 2) * 256];
+	const uint64_t g = blockIdx.y;
+	const uint64_t i_begin = g * a.in_per_group;
+	uint64_t i_end = i_begin + a.in_per_group;
+	if (i_end > a.N) i_end = a.N;
+	mac_tab_tile<RT, WPT, IB>(a, tab, (uint64_t)blockIdx.x * 256 * WPT, (uint64_t)blockIdx.z * RT,
+	                          i_begin, i_end, g, a.partial ? OUT_PARTIAL : OUT_DIRECT);
+}
+
+// --------------------------------------------------------------------------
+// Batched create: one persistent kernel drains a device queue of tiles that
+// span many independent jobs (uneven N, slice size, R).  Each work-group pulls
+// the next tile index with one atomic, runs the tile with the launch variant
+// the autotuner picked for its job, and loops until the queue is empty (every
+// wave of the group sees the same index, so all of them leave together).
+// --------------------------------------------------------------------------
+template <int RT, int WPT, int IB>
+__device__ __forceinline__ void run_batch_tile(const BatchJob& jb, const BatchTile& tl, uint4* tab) {
+	MacArgs a{};
+	a.in_base = jb.in;
+	a.in_stride = jb.in_stride;
+	a.out_base = jb.out;
+	a.out_stride = jb.out_stride;
+	a.coeff = jb.coeff;
+	a.coeff_ld = jb.N;
+	a.N = jb.N;
+	a.R = jb.R;
+	a.W = jb.W;
+	a.accumulate = 0;
+	mac_tab_tile<RT, WPT, IB>(a, tab, tl.w0, tl.r0, tl.i0, tl.i1, 0, tl.mode);
+}
+
+__global__ __launch_bounds__(256) void k_batch_persistent(const BatchJob* __restrict__ jobs,
+                                                          const BatchTile* __restrict__ tiles,
+                                                          uint32_t ntiles, uint32_t* counter) {
+	__shared__ uint4 tab[4096];  // 64 KiB: the largest IB*(RT/2)*256 of the variants
+	__shared__ uint32_t t_next;
+	for (;;) {
+		if (threadIdx.x == 0) t_next = atomicAdd(counter, 1u);
+		__syncthreads();
+		const uint32_t t = t_next;
+		__syncthreads();
+		if (t >= ntiles) break;
+		const BatchTile tl = tiles[t];
+		const BatchJob jb = jobs[tl.job];
+		switch (tl.cfg) {
+			case BATCH_CFG_8_4_4: run_batch_tile<8, 4, 4>(jb, tl, tab); break;
+			case BATCH_CFG_8_2_4: run_batch_tile<8, 2, 4>(jb, tl, tab); break;
+			case BATCH_CFG_16_2_2: run_batch_tile<16, 2, 2>(jb, tl, tab); break;
+			case BATCH_CFG_4_4_4: run_batch_tile<4, 4, 4>(jb, tl, tab); break;
+			default: break;
+		}
+	}
+}
+
+__global__ void k_reset_counter(uint32_t* counter) {
+	if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+// Cauchy matrices of every job of a batch: thread -> (job, r, c) through the
+// prefix offsets jobs[j].coeff_off (sorted), binary search over jobs.
+__global__ void k_cauchy_batch(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t total) {
+	for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+	     idx += (uint64_t)gridDim.x * blockDim.x) {
+		uint32_t lo = 0, hi = njobs - 1;
+		while (lo < hi) {
+			const uint32_t mid = (lo + hi + 1) >> 1;
+			if (jobs[mid].coeff_off <= idx) lo = mid; else hi = mid - 1;
+		}
+		const BatchJob& jb = jobs[lo];
+		const uint64_t e = idx - jb.coeff_off;
+		const uint64_t r = e / jb.N, c = e - r * jb.N;
+		const uint64_t d = (jb.fi + c) ^ (jb.fr + r);
+		jb.coeff[e] = gf64_inv(d ? d : 1);
+	}
+}
+
+// Zero the outputs of the jobs that are accumulated with atomics (a kernel,
+// not a memset node, so the sequence can be captured in a graph and replayed).
+__global__ void k_zero_outputs(const BatchJob* __restrict__ jobs, const uint32_t* __restrict__ ids,
+                               uint32_t nids) {
+	for (uint32_t k = blockIdx.y; k < nids; k += gridDim.y) {
+		const BatchJob& jb = jobs[ids[k]];
+		const uint64_t total = jb.R * jb.W;
+		for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+		     idx += (uint64_t)gridDim.x * blockDim.x) {
+			const uint64_t r = idx / jb.W, w = idx - r * jb.W;
+			jb.out[r * jb.out_stride + w] = 0;
 		}
 	}
 }
@@ -371,6 +473,27 @@ hipError_t launch_xor_reduce(uint64_t* partial, uint64_t G, uint64_t R, uint64_t
 	if (blocks > 16384) blocks = 16384;
 	hipLaunchKernelGGL(k_xor_reduce, dim3((unsigned)blocks), dim3(256), 0, s, src, G, R, W,
 	                   out_base, out_stride, out_ptrs, accumulate);
+	return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchJob* jobs, uint32_t njobs, const BatchTile* tiles,
+                        uint32_t ntiles, uint32_t* counter, const uint32_t* zero_ids,
+                        uint32_t nzero, uint64_t coeff_total, int blocks, hipStream_t s) {
+	if (coeff_total) {
+		uint64_t cb = (coeff_total + 255) / 256;
+		if (cb > 65536) cb = 65536;
+		hipLaunchKernelGGL(k_cauchy_batch, dim3((unsigned)cb), dim3(256), 0, s, jobs, njobs,
+		                   coeff_total);
+	}
+	if (nzero) {
+		hipLaunchKernelGGL(k_zero_outputs, dim3(64, nzero < 1024 ? nzero : 1024), dim3(256), 0, s,
+		                   jobs, zero_ids, nzero);
+	}
+	if (ntiles && blocks > 0) {
+		hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, counter);
+		hipLaunchKernelGGL(k_batch_persistent, dim3((unsigned)blocks), dim3(256), 0, s, jobs, tiles,
+		                   ntiles, counter);
+	}
 	return hipGetLastError();
 }
 

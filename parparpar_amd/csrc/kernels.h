@@ -23,6 +23,37 @@ struct MacArgs {
 	int accumulate;                   // XOR into existing outputs (when partial == nullptr)
 };
 
+// Batched create (persistent kernel over a device tile queue).
+enum BatchCfg : uint16_t {
+	BATCH_CFG_8_4_4 = 0,   // RT=8,  WPT=4, IB=4
+	BATCH_CFG_8_2_4 = 1,   // RT=8,  WPT=2, IB=4
+	BATCH_CFG_UNUSED_2 = 2,
+	BATCH_CFG_16_2_2 = 3,  // RT=16, WPT=2, IB=2
+	BATCH_CFG_4_4_4 = 4,   // RT=4,  WPT=4, IB=4
+	BATCH_CFG_COUNT = 5
+};
+struct BatchJob {
+	const uint64_t* in;
+	uint64_t in_stride;
+	uint64_t* out;
+	uint64_t out_stride;
+	uint64_t* coeff;      // R x N, built by the batch's Cauchy kernel
+	uint64_t N, R, W;
+	uint64_t fi, fr;
+	uint64_t coeff_off;   // first global index of this job's coefficients
+};
+struct BatchTile {
+	uint32_t job;
+	uint16_t cfg;         // BatchCfg
+	uint16_t mode;        // OUT_DIRECT (0) or OUT_ATOMIC (2)
+	uint64_t r0;
+	uint64_t w0, i0, i1;
+};
+
+hipError_t launch_batch(const BatchJob* jobs, uint32_t njobs, const BatchTile* tiles,
+                        uint32_t ntiles, uint32_t* counter, const uint32_t* zero_ids,
+                        uint32_t nzero, uint64_t coeff_total, int blocks, hipStream_t s);
+
 hipError_t launch_cauchy(uint64_t* out, uint64_t N, uint64_t R, uint64_t fi, uint64_t fr,
                          hipStream_t s);
 hipError_t launch_cauchy_points(uint64_t* out, const uint64_t* xs, uint64_t nx,
@@ -60,5 +91,13 @@ uint64_t gf16_reduce_scratch_words(uint64_t G, uint64_t R, uint64_t W);
 hipError_t launch_gf16_reduce(uint16_t* partial, uint64_t G, uint64_t R, uint64_t W,
                               uint16_t* out_base, uint64_t out_stride, int accumulate,
                               hipStream_t s);
+
+// Repair (closed-form Cauchy inverse and helpers).
+hipError_t launch_cauchy_inverse(const uint64_t* ys, const uint64_t* xs, uint64_t k,
+                                 uint64_t* alpha, uint64_t* beta, uint64_t* inv, hipStream_t s);
+hipError_t launch_repair_coeffs(uint64_t* out, uint64_t ld, const uint64_t* xs_known, uint64_t nk,
+                                const uint64_t* ys, uint64_t k, hipStream_t s);
+hipError_t launch_slice_ptrs(const uint64_t* base, uint64_t stride, const uint64_t* idx, uint64_t n,
+                             const uint64_t** out, hipStream_t s);
 
 }  // namespace parhip
