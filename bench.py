@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--slices", type=int, default=1024)
     ap.add_argument("--slice-bytes", type=int, default=1 << 20)
     ap.add_argument("--recovery", type=int, default=8)
+    ap.add_argument("--method", default="auto", choices=["auto", "table", "mfma", "mfma-i8"],
+                    help="contraction strategy: nibble tables in LDS, or the GF(2) bit-matrix "
+                         "on fp4 / int8 MFMA")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", default="par3-create",
@@ -147,6 +150,10 @@ def main_par3(args):
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     L = P.lib()
+    P.set_method({"auto": P.METHOD_AUTO, "table": P.METHOD_TABLE, "mfma": P.METHOD_MFMA,
+                  "mfma-i8": P.METHOD_MFMA_I8}[args.method])
+    mfma = args.method in ("mfma", "mfma-i8")
+    mac_name = "mac_mfma" if mfma else "mac_tab"
 
     def step():
         P.check(L.parhip_d_gf64_build_cauchy(d_coeff.data_ptr(), n, r, 0, n, sp))
@@ -173,9 +180,10 @@ def main_par3(args):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     P.prof_enable(False)
-    mac_ms, mac_n = P.prof_get("mac_tab")
+    mac_ms, mac_n = P.prof_get(mac_name)
     red_ms, red_n = P.prof_get("xor_reduce")
     cau_ms, cau_n = P.prof_get("cauchy")
+    rows_ms, rows_n = P.prof_get("mfma_rows")
 
     total_bytes = n * sb * args.steps
     value = total_bytes / 2 ** 30 / elapsed
@@ -185,7 +193,7 @@ def main_par3(args):
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_mac_tab_1024x1MiB_r8.txt")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and not mfma and (n, sb, r) == (1024, 1 << 20, 8):
         for line in open(tpath):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = int(line.split(":")[1].strip())
@@ -207,10 +215,12 @@ def main_par3(args):
             "workload": "PAR3 create 1 GiB = %d slices x %d KiB, R=%d, GF(2^64) Cauchy points (0, %d)"
                         % (n, sb // 1024, r, n),
             "slices": n, "slice_bytes": sb, "recovery": r, "parallelism": "single GPU",
-            "method": "nibble split tables in LDS (k_mac_tab) + split-K XOR reduce",
+            "method": {"mfma": "GF(2) bit-matrix on fp4 MFMA (k_mac_mfma)",
+                       "mfma-i8": "GF(2) bit-matrix on int8 MFMA, int32 parity (k_mac_mfma)"}.get(
+                args.method, "nibble split tables in LDS (k_mac_tab) + split-K XOR reduce"),
         },
         "kernels_ms_avg": {
-            "mac_tab": round(mac_ms / max(mac_n, 1), 4),
+            mac_name: round(mac_ms / max(mac_n, 1), 4),
             "xor_reduce": round(red_ms / max(red_n, 1), 4),
             "cauchy": round(cau_ms / max(cau_n, 1), 4),
         },
@@ -224,6 +234,15 @@ def main_par3(args):
         },
         "cpu_baseline": None,
     }
+    if mfma:
+        # the bit-matrix kernel is bound by the matrix cores: 2*64*64 bit-ops per
+        # (output, input, word); dense peaks 10.1 POP/s fp4, 5.0 POP/s int8
+        peak = 10100.0 if args.method == "mfma" else 5000.0
+        tops = 2.0 * 64 * 64 * n * r * w / mac_avg_s / 1e12
+        line["kernels_ms_avg"]["mfma_rows"] = round(rows_ms / max(rows_n, 1), 4)
+        line["roofline"] = {"bound": "mfma", "achieved": round(tops, 1), "peak": peak,
+                            "unit": "TFLOP/s", "frac": round(tops / peak, 4), "traffic": None,
+                            "hbm_gbs": round(achieved, 1)}
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(args, d_in, args.cpu_seconds)

@@ -44,7 +44,8 @@ extern "C" {
 /* ---- multiply strategies for the contraction ---------------------------- */
 #define PARHIP_METHOD_AUTO 0     /* per-shape choice (autotuner table)             */
 #define PARHIP_METHOD_TABLE 1    /* split nibble tables in LDS                     */
-#define PARHIP_METHOD_MFMA 2     /* GF(2) bit-matrix on MFMA                       */
+#define PARHIP_METHOD_MFMA 2     /* GF(2) bit-matrix on fp4 MFMA (exact 0/1 counts) */
+#define PARHIP_METHOD_MFMA_I8 3  /* GF(2) bit-matrix on int8 MFMA (int32 parity)    */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int parhip_version(void);
@@ -52,7 +53,8 @@ int parhip_version(void);
 const char *parhip_last_error(void);
 /* Select the HIP device used by calls from this thread (default 0). */
 int parhip_set_device(int device);
-/* Force a contraction strategy for this thread (PARHIP_METHOD_*). */
+/* Force a contraction strategy for this thread (PARHIP_METHOD_*). The MFMA
+ * strategies apply to strided slices; pointer-array calls use the tables. */
 int parhip_set_method(int method);
 /* Force the table-contraction launch shape for this thread (tuning; no
  * counterpart in the reference). rt = outputs per work-group, wpt = words per

@@ -102,6 +102,7 @@ PARHIP_ERR_RANGE = -5
 METHOD_AUTO = 0
 METHOD_TABLE = 1
 METHOD_MFMA = 2
+METHOD_MFMA_I8 = 3
 
 
 class ParhipError(RuntimeError):
@@ -128,6 +129,11 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def set_method(method):
+    """Contraction strategy for the calling thread (METHOD_*)."""
+    check(lib().parhip_set_method(int(method)))
 
 
 def prof_enable(on=True):

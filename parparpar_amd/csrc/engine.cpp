@@ -266,7 +266,15 @@ static thread_local int64_t t_tune_groups = 0;
 Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev) {
 	Plan p;
 	p.method = PARHIP_METHOD_TABLE;
-	p.rt = R >= 16 ? 16 : (R > 4 ? 8 : (R > 2 ? 4 : 2));
+	// output tile: the fewest padded rows, 8 on a tie.  Measured at 1024 x 1 MiB
+	// the cost per computed row is flat for 8- and 4-row tiles (~0.16 ms) and
+	// ~40% higher for 16-row tiles, so padding is what matters.
+	if (R <= 2) {
+		p.rt = 2;
+	} else {
+		const uint64_t pad8 = (R + 7) / 8 * 8, pad4 = (R + 3) / 4 * 4;
+		p.rt = pad8 <= pad4 ? 8 : 4;
+	}
 	p.wpt = W >= 1024 ? 4 : 2;
 	p.ib = p.rt <= 8 ? 4 : 2;
 	if (t_tune_rt || t_tune_wpt || t_tune_ib) {
@@ -308,11 +316,80 @@ size_t plan_workspace_bytes(const Plan& p, uint64_t R, uint64_t W) {
 	return p.groups > 1 ? (size_t)(xor_reduce_scratch_words(p.groups, R, W) * sizeof(uint64_t)) : 0;
 }
 
+// GF(2) bit-matrix on MFMA (gf64_mfma.hip): bit-matrix rows of every
+// coefficient prepared on the device, then the MFMA contraction, split-K
+// partials folded as for the table kernel.  Strided slices only.
+static int run_contraction_mfma(const MacArgs& m, bool fp4, hipStream_t s) {
+	// measured best at 1024 x 1 MiB, R = 8 and 32: fp4 4x2 tiles with 4-input
+	// stages; int8 (twice the fragments) 4x1 tiles with 2-input stages
+	int rt = 4, nb = fp4 ? 2 : 1, ib = fp4 ? 4 : 2;
+	if ((t_tune_rt || t_tune_wpt || t_tune_ib) &&
+	    mfma_supported(t_tune_rt ? t_tune_rt : rt, t_tune_wpt ? t_tune_wpt : nb,
+	                   t_tune_ib ? t_tune_ib : ib, fp4)) {
+		rt = t_tune_rt ? t_tune_rt : rt;
+		nb = t_tune_wpt ? t_tune_wpt : nb;
+		ib = t_tune_ib ? t_tune_ib : ib;
+	}
+	const uint64_t span = 4ull * 32 * nb;
+	const uint64_t base = ((m.W + span - 1) / span) * ((m.R + rt - 1) / rt);
+	const uint64_t target = (uint64_t)device_cus(t_device) * 2;
+	uint64_t groups = base < target ? (target + base - 1) / base : 1;
+	if (t_tune_groups > 0) groups = (uint64_t)t_tune_groups;
+	groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, m.N / 8));
+	// fp4 counts are fp32 sums of 0/1 products: exact while 64 * inputs < 2^24
+	if (fp4) groups = std::max<uint64_t>(groups, (m.N + (1u << 17) - 1) >> 17);
+	groups = std::min<uint64_t>(groups, 65535);
+	uint64_t per = (m.N + groups - 1) / groups;
+	groups = (m.N + per - 1) / per;
+	if (fp4 && per >= (1u << 18))
+		return fail(PARHIP_ERR_ARG, "mfma: %llu inputs per group exceed the exact fp32 count",
+		            (unsigned long long)per);
+	const size_t rows_bytes = (size_t)m.R * m.N * 64 * sizeof(uint64_t);
+	const size_t part_bytes =
+	    groups > 1 ? (size_t)xor_reduce_scratch_words(groups, m.R, m.W) * sizeof(uint64_t) : 0;
+	void* ws = nullptr;
+	int rc = dev_alloc(&ws, rows_bytes + part_bytes, s);
+	if (rc) return rc;
+	uint64_t* rows = (uint64_t*)ws;
+	hipError_t e;
+	{
+		ProfScope ps("mfma_rows", s);
+		e = launch_gf64_rows(m.coeff, m.coeff_ld, m.N, m.R, rows, s);
+	}
+	MfmaArgs a{};
+	a.in_base = m.in_base;
+	a.in_stride = m.in_stride;
+	a.out_base = m.out_base;
+	a.out_stride = m.out_stride;
+	a.rows = rows;
+	a.rows_ld = m.N;
+	a.N = m.N;
+	a.R = m.R;
+	a.W = m.W;
+	a.in_per_group = per;
+	a.partial = groups > 1 ? (uint64_t*)((char*)ws + rows_bytes) : nullptr;
+	a.accumulate = m.accumulate;
+	if (e == hipSuccess) {
+		ProfScope ps("mac_mfma", s);
+		e = launch_mac_mfma(a, groups, rt, nb, ib, fp4, s);
+	}
+	if (e == hipSuccess && groups > 1) {
+		ProfScope ps("xor_reduce", s);
+		e = launch_xor_reduce(a.partial, groups, m.R, m.W, m.out_base, m.out_stride, nullptr,
+		                      m.accumulate, s);
+	}
+	dev_free(ws, s);
+	return e == hipSuccess ? PARHIP_OK : hip_fail(e, "k_mac_mfma");
+}
+
 // Enqueue the full contraction described by `a` (partial/in_per_group are
 // filled in here) on stream s.
 int run_contraction(MacArgs a, hipStream_t s) {
 	if (a.N == 0 || a.R == 0) return PARHIP_OK;
 	if (a.W == 0) return PARHIP_OK;
+	if ((t_method == PARHIP_METHOD_MFMA || t_method == PARHIP_METHOD_MFMA_I8) && !a.in_ptrs &&
+	    !a.out_ptrs)
+		return run_contraction_mfma(a, t_method == PARHIP_METHOD_MFMA, s);
 	Plan p = plan_contraction(a.N, a.R, a.W, t_device);
 	a.in_per_group = p.in_per_group;
 	a.partial = nullptr;
@@ -361,7 +438,7 @@ int parhip_set_device(int device) {
 }
 
 int parhip_set_method(int method) {
-	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_MFMA)
+	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_MFMA_I8)
 		return fail(PARHIP_ERR_ARG, "parhip_set_method: unknown method %d", method);
 	t_method = method;
 	return PARHIP_OK;
@@ -388,7 +465,9 @@ int parhip_prof_get(const char* kernel, double* total_ms, uint64_t* launches) {
 }
 
 int parhip_tune_override(int rt, int wpt, int ib, int64_t groups) {
-	if ((rt || wpt || ib) && !mac_tab_supported(rt ? rt : 8, wpt ? wpt : 4, ib ? ib : 4))
+	if ((rt || wpt || ib) && !mac_tab_supported(rt ? rt : 8, wpt ? wpt : 4, ib ? ib : 4) &&
+	    !mfma_supported(rt ? rt : 4, wpt ? wpt : 2, ib ? ib : 2, true) &&
+	    !mfma_supported(rt ? rt : 4, wpt ? wpt : 2, ib ? ib : 2, false))
 		return fail(PARHIP_ERR_ARG, "tune_override: unsupported (rt=%d wpt=%d ib=%d)", rt, wpt, ib);
 	t_tune_rt = rt;
 	t_tune_wpt = wpt;

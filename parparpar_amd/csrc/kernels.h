@@ -92,6 +92,25 @@ hipError_t launch_gf16_reduce(uint16_t* partial, uint64_t G, uint64_t R, uint64_
                               uint16_t* out_base, uint64_t out_stride, int accumulate,
                               hipStream_t s);
 
+// GF(2) bit-matrix contraction on MFMA.
+struct MfmaArgs {
+	const uint64_t* in_base;
+	uint64_t in_stride;
+	uint64_t* out_base;
+	uint64_t out_stride;
+	const uint64_t* rows;  // (r*rows_ld + i)*64 + o : Row_o of coefficient (r, i)
+	uint64_t rows_ld;
+	uint64_t N, R, W;
+	uint64_t in_per_group;
+	uint64_t* partial;
+	int accumulate;
+};
+hipError_t launch_gf64_rows(const uint64_t* coeff, uint64_t ld, uint64_t N, uint64_t R,
+                            uint64_t* rows, hipStream_t s);
+hipError_t launch_mac_mfma(const MfmaArgs& a, uint64_t groups, int rt, int nb, int ib, bool fp4,
+                           hipStream_t s);
+bool mfma_supported(int rt, int nb, int ib, bool fp4);
+
 // Repair (closed-form Cauchy inverse and helpers).
 hipError_t launch_cauchy_inverse(const uint64_t* ys, const uint64_t* xs, uint64_t k,
                                  uint64_t* alpha, uint64_t* beta, uint64_t* inv, hipStream_t s);
