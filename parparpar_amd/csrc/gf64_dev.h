@@ -26,6 +26,22 @@ __host__ __device__ __forceinline__ uint64_t gf64_mulx_pow(uint64_t c, unsigned 
 	return gf64_reduce(c << j, c >> (64u - j));
 }
 
+// c * x^4: the 4 bits shifted out fold back as h * (x^4 + x^3 + x + 1), at
+// most 8 bits, so one fold suffices.
+__host__ __device__ __forceinline__ uint64_t gf64_mulx4(uint64_t c) {
+	const uint64_t h = c >> 60;
+	return (c << 4) ^ h ^ (h << 1) ^ (h << 3) ^ (h << 4);
+}
+
+// c * n(x) for a 4-bit polynomial n: at most 3 bits overflow, folded once.
+__host__ __device__ __forceinline__ uint64_t gf64_mul_nibble(uint64_t c, unsigned n) {
+	const uint64_t m0 = 0 - (uint64_t)(n & 1u), m1 = 0 - (uint64_t)((n >> 1) & 1u);
+	const uint64_t m2 = 0 - (uint64_t)((n >> 2) & 1u), m3 = 0 - (uint64_t)((n >> 3) & 1u);
+	const uint64_t lo = (c & m0) ^ ((c << 1) & m1) ^ ((c << 2) & m2) ^ ((c << 3) & m3);
+	const uint64_t h = ((c >> 63) & m1) ^ ((c >> 62) & m2) ^ ((c >> 61) & m3);
+	return lo ^ h ^ (h << 1) ^ (h << 3) ^ (h << 4);
+}
+
 // Full element product a*b: 4-bit windows over b, 16 precomputed multiples of a
 // kept as (lo, hi) 128-bit pairs, then one reduction.
 __host__ __device__ inline uint64_t gf64_mul(uint64_t a, uint64_t b) {

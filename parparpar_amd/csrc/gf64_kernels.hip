@@ -132,10 +132,15 @@ __device__ __forceinline__ void mac_tab_tile(const MacArgs& a, uint4* tab, uint6
 				d[ib][2 * j + 1] = make_uint2((uint32_t)v1, (uint32_t)(v1 >> 32));
 			}
 		}
-		// ---- build the split tables for IB inputs x RT outputs
+		// ---- build the split tables for IB inputs x RT outputs.  Thread
+		// (ib, p, n) owns entry n of every position k of its table pair:
+		// entry(k, n) = n(x) * c * x^{4k} = x^4 * entry(k-1, n), so it starts from
+		// the 4-bit product n*c and steps by x^4.  In each store the 16 lanes
+		// of a group write 16 consecutive entries of one 256-byte row, i.e. all
+		// 64 banks once: the build is free of LDS bank conflicts.
 		__syncthreads();  // previous batch finished reading tab
 		for (int t = tid; t < IB * NP * 16; t += 256) {
-			const int k = t & 15;
+			const int n = t & 15;
 			const int p = (t >> 4) % NP;
 			const int ib = (t >> 4) / NP;
 			const uint64_t i = ib0 + ib;
@@ -145,28 +150,19 @@ __device__ __forceinline__ void mac_tab_tile(const MacArgs& a, uint4* tab, uint6
 				if (ra < a.R) c0 = gload(a.coeff + ra * a.coeff_ld + i);
 				if (rb < a.R) c1 = gload(a.coeff + rb * a.coeff_ld + i);
 			}
-			uint64_t b0[4], b1[4];
+			uint64_t e0 = gf64_mul_nibble(c0, (unsigned)n);
+			uint64_t e1 = gf64_mul_nibble(c1, (unsigned)n);
 #pragma unroll
-			for (int q = 0; q < 4; q++) {
-				b0[q] = gf64_mulx_pow(c0, 4 * k + q);
-				b1[q] = gf64_mulx_pow(c1, 4 * k + q);
-			}
-			uint64_t e0[16], e1[16];
-			e0[0] = 0;
-			e1[0] = 0;
-#pragma unroll
-			for (int n = 1; n < 16; n++) {
-				const int low = __builtin_ctz(n);
-				e0[n] = e0[n & (n - 1)] ^ b0[low];
-				e1[n] = e1[n & (n - 1)] ^ b1[low];
-			}
-#pragma unroll
-			for (int n = 0; n < 16; n++) {
+			for (int k = 0; k < 16; k++) {
+				if (k) {
+					e0 = gf64_mulx4(e0);
+					e1 = gf64_mulx4(e1);
+				}
 				uint4 v;
-				v.x = (uint32_t)e0[n];
-				v.y = (uint32_t)(e0[n] >> 32);
-				v.z = (uint32_t)e1[n];
-				v.w = (uint32_t)(e1[n] >> 32);
+				v.x = (uint32_t)e0;
+				v.y = (uint32_t)(e0 >> 32);
+				v.z = (uint32_t)e1;
+				v.w = (uint32_t)(e1 >> 32);
 				TAB(ib, p, k, n) = v;
 			}
 		}
