@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", default="par3-create",
-                    choices=["par3-create", "par2-create", "batch", "repair"],
+                    choices=["par3-create", "par2-create", "batch", "repair", "host"],
                     help="par3-create (default line); the others are reported in DESIGN.md")
     ap.add_argument("--jobs", type=int, default=200)
     ap.add_argument("--rate", type=float, default=4.0, help="repair requests per second")
@@ -123,6 +123,8 @@ def main():
         return main_batch(args)
     if args.workload == "repair":
         return main_repair(args)
+    if args.workload == "host":
+        return main_host(args)
     return main_par3(args)
 
 
@@ -497,6 +499,58 @@ def main_repair(args):
         "achieved_req_per_s": round(nreq / wall, 3),
         "bit_exact_all_requests": bool(ok[0]),
         "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def main_host(args):
+    """Host-buffer create (PCIe-inclusive): the reference's callers hand over
+    host memory.  1 GiB pageable buffer -> parhip_gf64_compute_recovery_blocks
+    streamed through the pinned ring, versus one pageable copy then compute;
+    and the same bytes from a file via par3_create_streaming."""
+    import tempfile
+    import numpy as np
+    import torch
+    import parparpar_amd as P
+    from parparpar_amd import binding
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n, sb, r = args.slices, args.slice_bytes, args.recovery
+    w = sb // 8
+    rng = np.random.default_rng(0x5EED)
+    inp = np.frombuffer(rng.bytes(n * sb), dtype=np.uint64).reshape(n, w)
+    out = np.zeros((r, w), dtype=np.uint64)
+    res = {}
+    for name, chunk in (("streamed", 0), ("single_copy", n * sb + 1)):
+        binding.set_stream_chunk(chunk)
+        binding.compute_recovery(inp, out, n, r, sb, 0, n)  # warm: staging ring, pools
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            binding.compute_recovery(inp, out, n, r, sb, 0, n)
+        res[name] = n * sb * args.steps / 2 ** 30 / (time.perf_counter() - t0)
+        if name == "streamed":
+            ref = out.copy()
+        else:
+            assert np.array_equal(ref, out)
+    binding.set_stream_chunk(0)
+    with tempfile.NamedTemporaryFile(dir=os.environ.get("TMPDIR", "/tmp"), delete=True) as f:
+        inp.tofile(f.name)
+        opts = {"recoverySlices": r, "blockSize": sb, "firstInput": 0, "firstRecovery": n}
+        binding.par3_create_streaming(f.name, opts)  # warm, file in page cache
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fr = binding.par3_create_streaming(f.name, opts)
+        file_rate = n * sb * args.steps / 2 ** 30 / (time.perf_counter() - t0)
+        assert fr["recoveryBuffer"] == ref.tobytes()
+    line = {
+        "metric": "GiB/s encoded from host memory (PCIe-inclusive), 1GiB R=%d" % r,
+        "value": round(res["streamed"], 3), "unit": "GiB/s", "n_gpus": 1, "steps": args.steps,
+        "higher_is_better": True, "dtype": "u64",
+        "data": "synthetic (seeded uniform random bytes in pageable host memory / page-cached file)",
+        "config": {"workload": "host-pointer create %d x %d KiB, R=%d" % (n, sb // 1024, r)},
+        "streamed_GiBps": round(res["streamed"], 3),
+        "single_copy_GiBps": round(res["single_copy"], 3),
+        "file_streaming_GiBps": round(file_rate, 3),
+        "bit_exact": True,
     }
     print(json.dumps(line), flush=True)
 

@@ -60,7 +60,8 @@ int parhip_set_method(int method);
  * counterpart in the reference). rt = outputs per work-group, wpt = words per
  * thread, ib = inputs per LDS batch, groups = split-K rows; 0 = automatic. */
 int parhip_tune_override(int rt, int wpt, int ib, int64_t groups);
-/* Release cached device memory (coefficient cache, workspaces). */
+/* Release cached device memory (coefficient cache, workspaces, this thread's
+ * staging ring). */
 int parhip_release_cache(void);
 
 /* ======================================================================= */
@@ -120,6 +121,29 @@ int parhip_gf64_solve(const uint64_t *A, const uint64_t *b, uint64_t *x, size_t 
  * gf64/gf64_invert.c:29-67). Host computation. */
 uint64_t parhip_gf64_mul(uint64_t a, uint64_t b);
 uint64_t parhip_gf64_inverse(uint64_t a);
+
+/* Host inputs larger than one staging chunk are streamed to the device through
+ * a ring of pinned slots (copy threads + DMA + contraction overlapped, the
+ * chunks XOR-accumulated); results are identical to one contraction.
+ * Chunk size for this thread in bytes (0 = default 64 MiB, minimum 4096). */
+int parhip_set_stream_chunk(size_t bytes);
+
+/* par3_create_streaming (src/gf64_create_streaming.cc:93-240, N-API
+ * validation :327-470): read sourcePath, zero-pad the tail to whole blocks,
+ * numInputs = ceil(size / blockSize), compute every recovery block with the
+ * Cauchy points (firstInput, firstRecovery).  The file is read in chunks into
+ * pinned memory and reduced on the device while the next chunk is read.
+ * *recoveryOut: numRecovery * blockSize bytes from aligned_alloc(64) (free
+ * with free() or parhip_free); durationMs covers read + compute;
+ * throughputMBps = recovery MiB / s as the reference reports it.  useMmap and
+ * numThreads are accepted for signature parity. Errors: an open failure is
+ * reported with its Node-style code (ENOENT, EACCES, ...) in
+ * parhip_last_error(); an empty file and a blockSize that is not a positive
+ * multiple of 8 are rejected. */
+int parhip_par3_create_streaming(const char *sourcePath, int32_t numRecovery, int64_t blockSize,
+                                 uint64_t firstInput, uint64_t firstRecovery, int32_t numThreads,
+                                 int useMmap, uint8_t **recoveryOut, size_t *recoveryBytes,
+                                 double *durationMs, double *throughputMBps);
 
 /* ======================================================================= */
 /* Region operations (gf64/gf64_global.h:23-29), host pointers              */

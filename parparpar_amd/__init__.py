@@ -29,6 +29,11 @@ _SIGS = [
     ("parhip_set_device", _int, [_int]),
     ("parhip_set_method", _int, [_int]),
     ("parhip_release_cache", _int, []),
+    ("parhip_set_stream_chunk", _int, [_sz]),
+    ("parhip_par3_create_streaming", _int,
+     [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, _u64, _u64, ctypes.c_int32, _int,
+      ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)), ctypes.POINTER(_sz),
+      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     ("parhip_free", None, [_vp]),
     ("parhip_gf64_mul", _u64, [_u64, _u64]),
     ("parhip_gf64_inverse", _u64, [_u64]),

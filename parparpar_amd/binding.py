@@ -172,3 +172,42 @@ class Gf64Encoder:
             ctypes.addressof(optrs), len(outs), ctypes.addressof(iptrs), len(in_blocks),
             coeff2d.ctypes.data, K_stride, len_words))
         return outs
+
+
+def par3_create_streaming(sourcePath, options):
+    """par3_create_streaming(sourcePath, options) as the addon's callback result.
+
+    src/gf64_create_streaming.cc:327-470 (argument rules) and :93-240 (work):
+    options has recoverySlices, blockSize and optional firstInput,
+    firstRecovery, numThreads, useMmap.  Returns {recoveryBytes,
+    throughputMBps, durationMs, recoveryBuffer (bytes)}; raises on error with
+    the engine's message (Node-style errno code first for open failures).
+    """
+    if not isinstance(sourcePath, str) or not sourcePath:
+        raise TypeError("sourcePath must be a non-empty string")
+    if not isinstance(options, dict):
+        raise TypeError("options must be an object")
+    for k in ("recoverySlices", "blockSize"):
+        if k not in options:
+            raise TypeError("options.%s is required" % k)
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    nbytes = ctypes.c_size_t(0)
+    ms = ctypes.c_double(0)
+    mbps = ctypes.c_double(0)
+    L = lib()
+    check(L.parhip_par3_create_streaming(
+        sourcePath.encode(), int(options["recoverySlices"]), int(options["blockSize"]),
+        int(options.get("firstInput") or 0), int(options.get("firstRecovery") or 0),
+        int(options.get("numThreads") or 0), 1 if options.get("useMmap") else 0,
+        ctypes.byref(out), ctypes.byref(nbytes), ctypes.byref(ms), ctypes.byref(mbps)))
+    try:
+        buf = ctypes.string_at(out, nbytes.value)
+    finally:
+        L.parhip_free(ctypes.cast(out, ctypes.c_void_p))
+    return {"recoveryBytes": nbytes.value, "throughputMBps": mbps.value, "durationMs": ms.value,
+            "recoveryBuffer": buf}
+
+
+def set_stream_chunk(nbytes):
+    """Staging chunk of the host-pointer entries for this thread (0 = default)."""
+    check(lib().parhip_set_stream_chunk(int(nbytes)))

@@ -21,6 +21,7 @@
 
 #include "../../include/parhip.h"
 #include "engine_internal.h"
+#include "ingest.h"
 #include "gf64_host.h"
 #include "kernels.h"
 
@@ -478,6 +479,7 @@ int parhip_tune_override(int rt, int wpt, int ib, int64_t groups) {
 
 int parhip_release_cache(void) {
 	coeff_cache_clear();
+	ingest_release();
 	return PARHIP_OK;
 }
 

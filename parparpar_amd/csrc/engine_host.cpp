@@ -7,6 +7,14 @@
 // contraction (engine.cpp / gf64_kernels.hip) and copies the results back.
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cerrno>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -14,6 +22,7 @@
 #include "../../include/parhip.h"
 #include "engine_internal.h"
 #include "gf64_host.h"
+#include "ingest.h"
 #include "kernels.h"
 
 using namespace parhip;
@@ -95,6 +104,48 @@ static int host_contract(uint64_t* const* outs, size_t K, const uint64_t* const*
 	return sync(s);
 }
 
+// Contraction of N contiguous host input slices (W words each) against a
+// device coefficient matrix (R x N, leading dimension ld) into device outputs,
+// streamed through the pinned staging ring chunk by chunk.  The first chunk
+// writes (or XORs when `accumulate`), later chunks XOR: linearity makes it the
+// single contraction.
+static int stream_contract(const FillFn& fill, uint64_t N, uint64_t W, uint64_t* dout, uint64_t R,
+                           const uint64_t* dco, uint64_t ld, int accumulate, hipStream_t s) {
+	const ConsumeFn consume = [&](const void* src, uint64_t first, uint64_t count,
+	                              hipStream_t st) {
+		MacArgs a{};
+		a.in_base = (const uint64_t*)src;
+		a.in_stride = W;
+		a.out_base = dout;
+		a.out_stride = W;
+		a.coeff = dco + first;
+		a.coeff_ld = ld;
+		a.N = count;
+		a.R = R;
+		a.W = W;
+		a.accumulate = (accumulate || first > 0) ? 1 : 0;
+		return run_contraction(a, st);
+	};
+	return stream_slices(N, (size_t)W * sizeof(uint64_t), fill, consume, s);
+}
+
+// Parallel copy of `count` slices from a contiguous host buffer into a pinned slot.
+static FillFn memcpy_fill(const uint64_t* inputs, size_t slice_bytes) {
+	return [inputs, slice_bytes](void* dst, uint64_t first, uint64_t count) {
+		const char* src = (const char*)inputs + first * slice_bytes;
+		const size_t bytes = (size_t)count * slice_bytes, part = 4u << 20;
+		parallel_parts((bytes + part - 1) / part, [&](size_t p) {
+			const size_t o = p * part;
+			memcpy((char*)dst + o, src + o, std::min(part, bytes - o));
+		});
+		return PARHIP_OK;
+	};
+}
+
+// Inputs larger than one staging chunk are streamed (overlapped copies);
+// smaller ones take one copy.
+static bool use_streaming(size_t in_bytes) { return in_bytes > ingest_chunk_bytes(); }
+
 extern "C" {
 
 // ----------------------------------------------------------- GF64Controller
@@ -113,6 +164,18 @@ int parhip_gf64_compute_recovery_blocks_with_coeff(const uint64_t* inputs, size_
 	RC(words_bytes(numRecovery, numInputs, &cb));
 	hipStream_t s;
 	RC(host_stream(&s));
+	if (use_streaming(in_bytes)) {
+		DevBuf dout, dco;
+		RC(dout.alloc(out_bytes, s));
+		RC(dco.alloc(cb, s));
+		RC(h2d(dco.p, coeff, cb, s));
+		if (accumulate) RC(h2d(dout.p, recovery, out_bytes, s));
+		RC(stream_contract(memcpy_fill(inputs, blockSize64 * 8), numInputs, blockSize64,
+		                   dout.as<uint64_t>(), numRecovery, dco.as<uint64_t>(), numInputs, accumulate,
+		                   s));
+		RC(d2h(recovery, dout.p, out_bytes, s));
+		return sync(s);
+	}
 	DevBuf din, dout, dco;
 	RC(din.alloc(in_bytes, s));
 	RC(dout.alloc(out_bytes, s));
@@ -141,6 +204,21 @@ int parhip_gf64_compute_recovery_blocks(const uint64_t* inputs, size_t numInputs
 	RC(words_bytes(numRecovery, blockSize64, &out_bytes));
 	hipStream_t s;
 	RC(host_stream(&s));
+	if (use_streaming(in_bytes)) {
+		size_t cb;
+		RC(words_bytes(numRecovery, numInputs, &cb));
+		DevBuf dout, dco;
+		RC(dout.alloc(out_bytes, s));
+		RC(dco.alloc(cb, s));
+		RC(parhip_d_gf64_build_cauchy(dco.as<uint64_t>(), numInputs, numRecovery, firstInput,
+		                              firstRecovery, s));
+		if (accumulate) RC(h2d(dout.p, recovery, out_bytes, s));
+		RC(stream_contract(memcpy_fill(inputs, blockSize64 * 8), numInputs, blockSize64,
+		                   dout.as<uint64_t>(), numRecovery, dco.as<uint64_t>(), numInputs, accumulate,
+		                   s));
+		RC(d2h(recovery, dout.p, out_bytes, s));
+		return sync(s);
+	}
 	DevBuf din, dout;
 	RC(din.alloc(in_bytes, s));
 	RC(dout.alloc(out_bytes, s));
@@ -334,6 +412,116 @@ int parhip_gf64_region_2d_muladd_arr(uint64_t* const* outs, size_t K,
 	for (size_t g = 0; g < G; g++)
 		if (!in_blocks[g]) return fail(PARHIP_ERR_ARG, "region_2d_muladd_arr: null input");
 	return host_contract(outs, K, in_blocks, G, coeff_block_2d, K_stride, len, 1);
+}
+
+// ---------------------------------------------------------- streaming create
+// src/gf64_create_streaming.cc:93-240 (Par3csExecute): read the source file,
+// zero-pad the tail to a whole block, compute every recovery block with
+// ComputeRecoveryBlocksFull.  Here the file is read chunk by chunk straight
+// into pinned staging slots (pread on the host copy threads) and each chunk
+// is reduced on the device while the next one is read.
+static const char* errno_code(int err) {
+	switch (err) {
+		case ENOENT: return "ENOENT";
+		case EACCES: return "EACCES";
+		case EISDIR: return "EISDIR";
+		case EFBIG: return "EFBIG";
+		case ENOMEM: return "ENOMEM";
+		case ENAMETOOLONG: return "ENAMETOOLONG";
+		default: return "EIO";
+	}
+}
+
+int parhip_par3_create_streaming(const char* sourcePath, int32_t numRecovery, int64_t blockSize,
+                                 uint64_t firstInput, uint64_t firstRecovery, int32_t numThreads,
+                                 int useMmap, uint8_t** recoveryOut, size_t* recoveryBytes,
+                                 double* durationMs, double* throughputMBps) {
+	(void)numThreads;
+	(void)useMmap;  // the data path is pread -> pinned -> HBM either way
+	if (!recoveryOut) return fail(PARHIP_ERR_ARG, "par3_create_streaming: null result pointer");
+	*recoveryOut = nullptr;
+	if (recoveryBytes) *recoveryBytes = 0;
+	if (!sourcePath || !sourcePath[0])
+		return fail(PARHIP_ERR_ARG, "sourcePath must be a non-empty string");
+	if (numRecovery <= 0) return fail(PARHIP_ERR_RANGE, "recoverySlices must be a positive integer");
+	if (blockSize <= 0 || blockSize % 8 != 0)
+		return fail(PARHIP_ERR_RANGE, "blockSize must be positive and a multiple of 8");
+	const int fd = ::open(sourcePath, O_RDONLY);
+	if (fd < 0) {
+		const int err = errno;
+		return fail(PARHIP_ERR_ARG, "%s: open failed for sourcePath: errno=%d (%s)", errno_code(err),
+		            err, strerror(err));
+	}
+	struct stat st;
+	if (::fstat(fd, &st) != 0) {
+		const int err = errno;
+		::close(fd);
+		return fail(PARHIP_ERR_ARG, "fstat failed: errno=%d (%s)", err, strerror(err));
+	}
+	if (st.st_size <= 0) {
+		::close(fd);
+		return fail(PARHIP_ERR_ARG, "sourcePath is empty");
+	}
+	const size_t total = (size_t)st.st_size, S = (size_t)blockSize, W = S / 8;
+	const uint64_t N = (total + S - 1) / S;
+	const size_t R = (size_t)numRecovery;
+	size_t out_bytes, cb;
+	int rc = words_bytes(R, W, &out_bytes);
+	if (!rc) rc = words_bytes(R, N, &cb);
+	uint8_t* rec = nullptr;
+	if (!rc) {
+		rec = (uint8_t*)aligned_alloc(64, (out_bytes + 63) & ~(size_t)63);
+		if (!rec) rc = fail(PARHIP_ERR_NOMEM, "aligned_alloc failed for recovery buffer");
+	}
+	const auto t0 = std::chrono::steady_clock::now();
+	hipStream_t s = nullptr;
+	if (!rc) rc = host_stream(&s);
+	if (!rc) {
+		DevBuf dout, dco;
+		rc = dout.alloc(out_bytes, s);
+		if (!rc) rc = dco.alloc(cb, s);
+		if (!rc)
+			rc = parhip_d_gf64_build_cauchy(dco.as<uint64_t>(), N, R, firstInput, firstRecovery, s);
+		std::atomic<int> read_err{0};
+		const FillFn fill = [&](void* dst, uint64_t first, uint64_t count) {
+			const size_t off = (size_t)first * S, bytes = (size_t)count * S, part = 4u << 20;
+			parallel_parts((bytes + part - 1) / part, [&](size_t p) {
+				const size_t o = p * part, len = std::min(part, bytes - o);
+				size_t got = 0;
+				while (got < len && off + o + got < total) {
+					const size_t want = std::min(len - got, total - (off + o + got));
+					const ssize_t r = ::pread(fd, (char*)dst + o + got, want, (off_t)(off + o + got));
+					if (r < 0 && errno == EINTR) continue;
+					if (r <= 0) {
+						int expect = 0;
+						read_err.compare_exchange_strong(expect, r < 0 ? errno : EIO);
+						return;
+					}
+					got += (size_t)r;
+				}
+				if (got < len) memset((char*)dst + o + got, 0, len - got);  // zero tail
+			});
+			const int err = read_err.load();
+			if (err)
+				return fail(PARHIP_ERR_ARG, "read failed: errno=%d (%s)", err, strerror(err));
+			return (int)PARHIP_OK;
+		};
+		if (!rc) rc = stream_contract(fill, N, W, dout.as<uint64_t>(), R, dco.as<uint64_t>(), N, 0, s);
+		if (!rc) rc = d2h(rec, dout.p, out_bytes, s);
+		if (!rc) rc = sync(s);
+	}
+	::close(fd);
+	if (rc) {
+		free(rec);
+		return rc;
+	}
+	const double ms =
+	    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+	*recoveryOut = rec;
+	if (recoveryBytes) *recoveryBytes = out_bytes;
+	if (durationMs) *durationMs = ms;
+	if (throughputMBps) *throughputMBps = ms > 0 ? ((double)out_bytes / 1048576.0) / (ms / 1000.0) : 0;
+	return PARHIP_OK;
 }
 
 }  // extern "C"
