@@ -186,6 +186,16 @@ def main_par3(args):
     red_ms, red_n = P.prof_get("xor_reduce")
     cau_ms, cau_n = P.prof_get("cauchy")
     rows_ms, rows_n = P.prof_get("mfma_rows")
+    # the step after create: R sealed PAR REC packets from the recovery blocks in HBM
+    d_pk = torch.empty((r, 64 + sb), dtype=torch.uint8, device="cuda")
+    isid = bytes(range(8))
+    P.prof_enable(True)
+    P.prof_reset()
+    for _ in range(3):
+        P.check(L.parhip_d_assemble_recovery_packets(d_out.data_ptr(), isid, n, r, sb, d_pk.data_ptr(), sp))
+    torch.cuda.synchronize()
+    pk_ms, pk_n = P.prof_get("rec_packets")
+    P.prof_enable(False)
 
     total_bytes = n * sb * args.steps
     value = total_bytes / 2 ** 30 / elapsed
@@ -226,6 +236,7 @@ def main_par3(args):
             "xor_reduce": round(red_ms / max(red_n, 1), 4),
             "cauchy": round(cau_ms / max(cau_n, 1), 4),
         },
+        "rec_packets_ms": round(pk_ms / max(pk_n, 1), 4),
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),

@@ -146,6 +146,27 @@ int parhip_par3_create_streaming(const char *sourcePath, int32_t numRecovery, in
                                  double *durationMs, double *throughputMBps);
 
 /* ======================================================================= */
+/* PAR3 recovery packets                                                    */
+/* ======================================================================= */
+
+/* assemble_recovery_packets (AssembleRecoveryPackets_NAPI,
+ * src/gf64_addon.cc:2790-2938): numRecovery packets of 64 + blockSize bytes
+ * into out (caller-allocated, numRecovery * (64 + blockSize) bytes): magic
+ * "PAR3\0PKT", 16-byte BLAKE3 of packet[24..end), LE64 packet length, the
+ * 8-byte inputSetId, "PAR REC\0", LE64 block index firstRecoveryIndex + i,
+ * LE64 count 1, block i of `recovery` (numRecovery * blockSize bytes).
+ * Assembled and hashed on the device.  numRecovery and blockSize must be
+ * positive and their product at most 1 TiB (PARHIP_ERR_RANGE otherwise). */
+int parhip_assemble_recovery_packets(const uint8_t *recovery, const uint8_t *inputSetId,
+                                     int64_t firstRecoveryIndex, int64_t numRecovery,
+                                     int64_t blockSize, uint8_t *out);
+/* Device-pointer form (inputSetId stays a host pointer to 8 bytes); enqueues
+ * on `stream` without synchronising. */
+int parhip_d_assemble_recovery_packets(const void *d_recovery, const uint8_t *inputSetId,
+                                       int64_t firstRecoveryIndex, int64_t numRecovery,
+                                       int64_t blockSize, void *d_out, void *stream);
+
+/* ======================================================================= */
 /* Region operations (gf64/gf64_global.h:23-29), host pointers              */
 /* ======================================================================= */
 /* out[w] = in[w] * c */

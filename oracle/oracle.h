@@ -72,6 +72,14 @@ void or_par2_recovery_columns(const uint8_t *inputs, size_t numInputs, size_t sl
                               size_t numOutputs, const uint64_t *cols, size_t ncols,
                               uint16_t *out);
 
+/* ---------------- PAR3 recovery packets + BLAKE3 seal ---------------- */
+void or_blake3_compress(const uint32_t cv[8], const uint32_t block[16], uint64_t counter,
+                        uint32_t block_len, uint32_t flags, uint32_t out[16]);
+void or_blake3(const uint8_t *msg, size_t len, uint8_t out[32]);
+void or_assemble_recovery_packets(const uint8_t *recovery, const uint8_t inputSetId[8],
+                                  uint64_t firstRecoveryIndex, size_t numRecovery,
+                                  size_t blockSize, uint8_t *out);
+
 /* ---------------- helpers used by tests ---------------- */
 uint32_t or_crc32(const uint8_t *buf, size_t len);
 

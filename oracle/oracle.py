@@ -76,6 +76,8 @@ def lib():
         L.or_par2_recovery_columns.argtypes = [_u8p, _sz, _sz, _u16p, _u16p, _sz, _u64p, _sz, _u16p]
         L.or_crc32.restype = ctypes.c_uint32
         L.or_crc32.argtypes = [_u8p, _sz]
+        L.or_blake3.argtypes = [_u8p, _sz, _u8p]
+        L.or_assemble_recovery_packets.argtypes = [_u8p, _u8p, _u64, _sz, _sz, _u8p]
         _lib = L
     return _lib
 
@@ -101,6 +103,7 @@ def ref():
         R.ref_compute_recovery.argtypes = [_u64p, _sz, _u64p, _sz, _sz, _u64, _u64, ctypes.c_int, ctypes.c_int]
         R.ref_compute_recovery_with_coeff.argtypes = [_u64p, _sz, _u64p, _sz, _sz, _u64p, ctypes.c_int,
                                                       ctypes.c_int]
+        R.ref_blake3_2.argtypes = [_u8p, _sz, _u8p, _sz, _u8p]
         R.ref_solve_and_reconstruct.restype = ctypes.c_int
         R.ref_solve_and_reconstruct.argtypes = [_u64p, _u64p, _sz, _sz, ctypes.c_int]
         R.gfmat_init.restype = None
@@ -215,3 +218,35 @@ def par2_create(inputs, input_nums, exponents, slice_size):
             L.or_gf16_muladd_region(row.ctypes.data_as(_u8p), p8(b), b.size,
                                     L.or_gf16_coeff(int(input_nums[i]), int(e)))
     return np.ascontiguousarray(outs[:, :slice_size])
+
+
+def _bytes_ptr(b):
+    a = np.frombuffer(bytes(b), dtype=np.uint8) if len(b) else np.zeros(1, dtype=np.uint8)
+    return a, a.ctypes.data_as(_u8p)
+
+
+def blake3(data):
+    """32-byte BLAKE3 digest of `data` (bytes)."""
+    a, p = _bytes_ptr(data)
+    out = np.zeros(32, dtype=np.uint8)
+    lib().or_blake3(p, len(data), out.ctypes.data_as(_u8p))
+    return out.tobytes()
+
+
+def ref_blake3(a, b=b""):
+    """The reference's BLAKE3 over a || b (oracle/_ref), as its packetizer calls it."""
+    x, px = _bytes_ptr(a)
+    y, py = _bytes_ptr(b)
+    out = np.zeros(32, dtype=np.uint8)
+    ref().ref_blake3_2(px, len(a), py, len(b), out.ctypes.data_as(_u8p))
+    return out.tobytes()
+
+
+def assemble_recovery_packets(recovery, input_set_id, first_index, num, block_size):
+    """numRecovery PAR3 REC packets of 64 + block_size bytes each."""
+    assert len(recovery) == num * block_size and len(input_set_id) == 8
+    r, pr = _bytes_ptr(recovery)
+    i, pi = _bytes_ptr(input_set_id)
+    out = np.zeros(num * (64 + block_size), dtype=np.uint8)
+    lib().or_assemble_recovery_packets(pr, pi, first_index, num, block_size, out.ctypes.data_as(_u8p))
+    return out.tobytes()

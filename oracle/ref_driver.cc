@@ -9,6 +9,7 @@
 #include <cstddef>
 #include <cstdint>
 #include "par3_engine.h"
+#include "../hasher/hasher_blake3.h"
 
 extern "C" {
 uint64_t gf64_mul_reference(uint64_t a, uint64_t b);
@@ -40,5 +41,15 @@ void ref_compute_recovery_with_coeff(const uint64_t *in, size_t n, uint64_t *rec
 
 int ref_solve_and_reconstruct(uint64_t *A, uint64_t *rhs, size_t n, size_t bs, int threads) {
 	return GF64Controller::SolveAndReconstruct(A, rhs, n, bs, threads);
+}
+
+/* The reference's BLAKE3 (hasher/hasher_blake3.c), as its packetizer calls it
+ * (src/gf64_addon.cc:2928-2933): two updates, 32-byte finalize. */
+void ref_blake3_2(const uint8_t *a, size_t alen, const uint8_t *b, size_t blen, uint8_t *out32) {
+	blake3_hasher h;
+	blake3_hasher_init(&h);
+	blake3_hasher_update(&h, a, alen);
+	blake3_hasher_update(&h, b, blen);
+	blake3_hasher_finalize(&h, out32, 32);
 }
 }

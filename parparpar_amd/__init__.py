@@ -30,6 +30,10 @@ _SIGS = [
     ("parhip_set_method", _int, [_int]),
     ("parhip_release_cache", _int, []),
     ("parhip_set_stream_chunk", _int, [_sz]),
+    ("parhip_assemble_recovery_packets", _int,
+     [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp]),
+    ("parhip_d_assemble_recovery_packets", _int,
+     [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp, _vp]),
     ("parhip_par3_create_streaming", _int,
      [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, _u64, _u64, ctypes.c_int32, _int,
       ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)), ctypes.POINTER(_sz),

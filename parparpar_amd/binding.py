@@ -211,3 +211,23 @@ def par3_create_streaming(sourcePath, options):
 def set_stream_chunk(nbytes):
     """Staging chunk of the host-pointer entries for this thread (0 = default)."""
     check(lib().parhip_set_stream_chunk(int(nbytes)))
+
+
+def assemble_recovery_packets(recoveryBlocksBuf, inputSetIdBuf, firstRecoveryIndex, numRecovery,
+                              blockSize):
+    """assemble_recovery_packets (src/gf64_addon.cc:2790-2938): bytes of
+    numRecovery * (64 + blockSize) PAR3 REC packets, sealed with BLAKE3."""
+    rec = bytes(recoveryBlocksBuf)
+    isid = bytes(inputSetIdBuf)
+    if len(isid) != 8:
+        raise TypeError("inputSetIdBuf must be a 8-byte Buffer")
+    if numRecovery <= 0:
+        raise ValueError("numRecovery must be a positive integer")
+    if blockSize <= 0:
+        raise ValueError("blockSize must be a positive integer")
+    if len(rec) != numRecovery * blockSize:
+        raise ValueError("recoveryBlocksBuf length must equal numRecovery * blockSize")
+    out = ctypes.create_string_buffer(numRecovery * (64 + blockSize))
+    check(lib().parhip_assemble_recovery_packets(rec, isid, int(firstRecoveryIndex), numRecovery,
+                                                 blockSize, out))
+    return out.raw

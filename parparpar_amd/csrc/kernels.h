@@ -111,6 +111,11 @@ hipError_t launch_mac_mfma(const MfmaArgs& a, uint64_t groups, int rt, int nb, i
                            hipStream_t s);
 bool mfma_supported(int rt, int nb, int ib, bool fp4);
 
+// PAR3 recovery packets (packets.hip): ws holds 2 x num x chunks x 8 words.
+uint64_t b3_chunks_of(uint64_t block_size);
+hipError_t launch_rec_packets(const uint8_t* rec, uint64_t isid, uint64_t first, uint64_t num,
+                              uint64_t bs, uint8_t* out, uint32_t* ws, hipStream_t s);
+
 // Repair (closed-form Cauchy inverse and helpers).
 hipError_t launch_cauchy_inverse(const uint64_t* ys, const uint64_t* xs, uint64_t k,
                                  uint64_t* alpha, uint64_t* beta, uint64_t* inv, hipStream_t s);

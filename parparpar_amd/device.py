@@ -100,3 +100,17 @@ def gf16_muladd(inputs, coeff, out=None, words=None, accumulate=False, stream=No
                                      out.stride(0), words, coeff.data_ptr(), 1 if accumulate else 0,
                                      _stream(stream)))
     return out
+
+
+def assemble_recovery_packets(recovery, input_set_id, first_index, out=None, stream=None):
+    """PAR3 REC packets from a (R, S) uint8 device tensor of recovery blocks:
+    returns a (R, 64 + S) uint8 device tensor (enqueued on `stream`)."""
+    r, bs = recovery.shape
+    if recovery.dtype != torch.uint8 or not recovery.is_contiguous():
+        raise TypeError("recovery must be a contiguous uint8 tensor")
+    if out is None:
+        out = torch.empty((r, 64 + bs), dtype=torch.uint8, device=recovery.device)
+    isid = bytes(input_set_id)
+    check(lib().parhip_d_assemble_recovery_packets(recovery.data_ptr(), isid, int(first_index), r, bs,
+                                                   out.data_ptr(), _stream(stream)))
+    return out
