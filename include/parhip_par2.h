@@ -39,6 +39,28 @@ int parhip_d_gf16_muladd(const uint16_t *d_inputs, size_t numInputs, size_t inpu
                          uint16_t *d_outputs, size_t numOutputs, size_t outputStrideWords,
                          size_t words, const uint16_t *d_coeff, int accumulate, void *stream);
 
+/* ---- PAR2 repair matrix ------------------------------------------------- */
+/* Galois16RecMatrix::Compute (gf16/gfmat_inv.cpp:562-676, Construct :462-560):
+ * inputValid[i] != 0 for the inputs present (validCount of numInputs <= 32768);
+ * recovery[0..*recoveryCount) the available exponents.  The exponents are
+ * sorted; a recovery row whose pivot turns out zero (PAR2's non-invertible
+ * combinations) is dropped and the inversion retried.  Returns 1 when
+ * numRec = numInputs - validCount rows were inverted: recovery[0..numRec) are
+ * the exponents used, *recoveryCount = numRec, and factors (numRec x
+ * numInputs, row-major) hold GetFactor(col, row): columns 0..validCount-1 the
+ * valid inputs in input order, then the missing inputs in input order.
+ * Returns 0 (Compute's false) when too few exponents remain (*recoveryCount =
+ * those left), a negative PARHIP_ERR_* on invalid arguments or device errors.
+ * The inversion runs on the device. */
+int parhip_gf16_rec_matrix_compute(const uint8_t *inputValid, uint32_t numInputs,
+                                   uint32_t validCount, uint16_t *recovery,
+                                   uint32_t *recoveryCount, uint16_t *factors);
+/* Same, factors left in device memory (numRec x numInputs uint16) for a
+ * device repair contraction; synchronises `stream` once per attempt. */
+int parhip_d_gf16_rec_matrix_compute(const uint8_t *inputValid, uint32_t numInputs,
+                                     uint32_t validCount, uint16_t *recovery,
+                                     uint32_t *recoveryCount, uint16_t *d_factors, void *stream);
+
 /* ---- PAR2ProcHIP backend (IPAR2ProcBackend contract) -------------------- */
 typedef struct parhip_par2 parhip_par2;
 

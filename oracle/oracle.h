@@ -67,6 +67,8 @@ void or_gf16_muladd_region(uint8_t *out, const uint8_t *in, size_t bytes, uint16
 void or_par2_create(const uint8_t *const *inputs, const size_t *sizes, const uint16_t *inputNums,
                     size_t numInputs, uint8_t *const *outputs, const uint16_t *exponents,
                     size_t numOutputs, size_t sliceSize);
+int or_gf16_rec_matrix_compute(const uint8_t *inputValid, size_t numInputs, size_t validCount,
+                               uint16_t *recovery, size_t *recoveryCount, uint16_t *out);
 void or_par2_recovery_columns(const uint8_t *inputs, size_t numInputs, size_t sliceSize,
                               const uint16_t *inputNums, const uint16_t *exponents,
                               size_t numOutputs, const uint64_t *cols, size_t ncols,

@@ -77,6 +77,8 @@ def lib():
         L.or_crc32.restype = ctypes.c_uint32
         L.or_crc32.argtypes = [_u8p, _sz]
         L.or_blake3.argtypes = [_u8p, _sz, _u8p]
+        L.or_gf16_rec_matrix_compute.restype = ctypes.c_int
+        L.or_gf16_rec_matrix_compute.argtypes = [_u8p, _sz, _sz, _u16p, ctypes.POINTER(_sz), _u16p]
         L.or_assemble_recovery_packets.argtypes = [_u8p, _u8p, _u64, _sz, _sz, _u8p]
         _lib = L
     return _lib
@@ -250,3 +252,16 @@ def assemble_recovery_packets(recovery, input_set_id, first_index, num, block_si
     out = np.zeros(num * (64 + block_size), dtype=np.uint8)
     lib().or_assemble_recovery_packets(pr, pi, first_index, num, block_size, out.ctypes.data_as(_u8p))
     return out.tobytes()
+
+
+def gf16_rec_matrix(input_valid, recovery):
+    """Galois16RecMatrix::Compute: (ok, used exponents, factors[numRec, numInputs])."""
+    valid = np.ascontiguousarray(np.asarray(input_valid, dtype=np.uint8))
+    n = valid.size
+    vc = int(valid.sum())
+    rec = np.ascontiguousarray(np.asarray(recovery, dtype=np.uint16)).copy()
+    cnt = ctypes.c_size_t(rec.size)
+    out = np.zeros((n - vc, n), dtype=np.uint16)
+    ok = lib().or_gf16_rec_matrix_compute(valid.ctypes.data_as(_u8p), n, vc, p16(rec),
+                                          ctypes.byref(cnt), p16(out))
+    return bool(ok), rec[:cnt.value].copy(), out

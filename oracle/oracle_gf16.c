@@ -129,3 +129,77 @@ void or_par2_recovery_columns(const uint8_t *inputs, size_t numInputs, size_t sl
 			out[o * ncols + j] = s;
 		}
 }
+
+/* ---------------------------------------------------------------------------
+ * PAR2 repair matrix: Galois16RecMatrix::Compute (gf16/gfmat_inv.cpp:562-676)
+ * with Construct (:462-560) and the in-place Gauss-Jordan of scaleRows /
+ * applyRows (:130-300), one row at a time.
+ *
+ * Columns: the valid inputs in input order, then the missing inputs in input
+ * order (Construct's validCol / missingCol).  Row r holds recovery exponent
+ * recovery[r] (sorted ascending first).  Pivot r sits at column validCount + r
+ * and is never swapped; its entry is replaced by 1 before the row is scaled by
+ * the pivot's reciprocal, and every other row's entry in that column is
+ * replaced by 0 before the row gets factor * pivot row (REPLACE_WORD), which
+ * leaves the inverse of the missing block in the missing columns and
+ * inverse * valid block in the valid columns.  A zero pivot (PAR2's
+ * non-invertible combinations) drops that recovery exponent and restarts; too
+ * few exponents left -> 0.  On success the exponents are the numRec used and
+ * out[r * numInputs + col] is GetFactor(col, r).
+ * ------------------------------------------------------------------------- */
+int or_gf16_rec_matrix_compute(const uint8_t *inputValid, size_t numInputs, size_t validCount,
+                               uint16_t *recovery, size_t *recoveryCount, uint16_t *out) {
+	const size_t numRec = numInputs - validCount;
+	size_t cnt = *recoveryCount;
+	/* std::sort(recovery) (gfmat_inv.cpp:594) */
+	for (size_t i = 1; i < cnt; i++) {
+		uint16_t v = recovery[i];
+		size_t j = i;
+		while (j > 0 && recovery[j - 1] > v) {
+			recovery[j] = recovery[j - 1];
+			j--;
+		}
+		recovery[j] = v;
+	}
+	for (;;) {
+		if (numRec > cnt) {
+			*recoveryCount = cnt;
+			return 0;
+		}
+		/* Construct */
+		size_t validCol = 0, missingCol = validCount;
+		for (size_t i = 0; i < numInputs; i++) {
+			const size_t col = inputValid[i] ? validCol++ : missingCol++;
+			for (size_t r = 0; r < numRec; r++)
+				out[r * numInputs + col] = or_gf16_coeff((uint32_t)i, recovery[r]);
+		}
+		/* eliminate */
+		long bad = -1;
+		for (size_t k = 0; k < numRec && bad < 0; k++) {
+			uint16_t *row = out + k * numInputs;
+			const size_t pc = validCount + k;
+			const uint16_t p = row[pc];
+			if (p == 0) {
+				bad = (long)k;
+				break;
+			}
+			row[pc] = 1;
+			const uint16_t ip = or_gf16_exp(65535u - or_gf16_log(p));
+			for (size_t c = 0; c < numInputs; c++) row[c] = or_gf16_mul(row[c], ip);
+			for (size_t r = 0; r < numRec; r++) {
+				if (r == k) continue;
+				uint16_t *dst = out + r * numInputs;
+				const uint16_t t = dst[pc];
+				dst[pc] = 0;
+				if (t == 0) continue;
+				for (size_t c = 0; c < numInputs; c++) dst[c] ^= or_gf16_mul(t, row[c]);
+			}
+		}
+		if (bad < 0) break;
+		/* recovery.erase(recovery.begin() + badRow); goto invert_loop (:638-646) */
+		for (size_t i = (size_t)bad; i + 1 < cnt; i++) recovery[i] = recovery[i + 1];
+		cnt--;
+	}
+	*recoveryCount = numRec; /* recovery.resize(numRec) */
+	return 1;
+}

@@ -30,6 +30,10 @@ _SIGS = [
     ("parhip_set_method", _int, [_int]),
     ("parhip_release_cache", _int, []),
     ("parhip_set_stream_chunk", _int, [_sz]),
+    ("parhip_gf16_rec_matrix_compute", _int,
+     [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.POINTER(ctypes.c_uint32), _vp]),
+    ("parhip_d_gf16_rec_matrix_compute", _int,
+     [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.POINTER(ctypes.c_uint32), _vp, _vp]),
     ("parhip_assemble_recovery_packets", _int,
      [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp]),
     ("parhip_d_assemble_recovery_packets", _int,

@@ -21,6 +21,7 @@ namespace parhip {
 struct Gf16DevTables {
 	uint16_t* exp_tab = nullptr;
 	uint16_t* inlog = nullptr;
+	uint16_t* log_tab = nullptr;
 };
 static std::mutex g_t16_mu;
 static std::map<int, Gf16DevTables> g_t16;
@@ -35,11 +36,23 @@ int gf16_device_tables(const uint16_t** exp_tab, const uint16_t** inlog) {
 		if (e != hipSuccess) return hip_fail(e, "hipMalloc(gf16 tables)");
 		e = hipMalloc((void**)&t.inlog, 32768 * sizeof(uint16_t));
 		if (e != hipSuccess) return hip_fail(e, "hipMalloc(gf16 tables)");
+		e = hipMalloc((void**)&t.log_tab, 65536 * sizeof(uint16_t));
+		if (e != hipSuccess) return hip_fail(e, "hipMalloc(gf16 tables)");
 		hipMemcpy(t.exp_tab, h.exp, 65536 * sizeof(uint16_t), hipMemcpyHostToDevice);
 		hipMemcpy(t.inlog, h.inlog, 32768 * sizeof(uint16_t), hipMemcpyHostToDevice);
+		hipMemcpy(t.log_tab, h.log, 65536 * sizeof(uint16_t), hipMemcpyHostToDevice);
 	}
 	*exp_tab = t.exp_tab;
 	*inlog = t.inlog;
+	return PARHIP_OK;
+}
+
+int gf16_device_log_exp(const uint16_t** exp_tab, const uint16_t** log_tab) {
+	const uint16_t* il;
+	int rc = gf16_device_tables(exp_tab, &il);
+	if (rc) return rc;
+	std::lock_guard<std::mutex> lk(g_t16_mu);
+	*log_tab = g_t16[current_device()].log_tab;
 	return PARHIP_OK;
 }
 

@@ -30,6 +30,12 @@ Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev);
 size_t plan_workspace_bytes(const Plan& p, uint64_t R, uint64_t W);
 int run_contraction(MacArgs a, hipStream_t s);
 
+// PAR2 GF(2^16) side (engine_gf16.cpp)
+int gf16_device_log_exp(const uint16_t** exp_tab, const uint16_t** log_tab);
+int gf16_coeffs_device(uint16_t* d_coeff, const uint16_t* d_input_nums, uint64_t N,
+                       const uint16_t* d_exps, uint64_t R, hipStream_t s);
+int run_gf16_contraction(Gf16Args a, hipStream_t s);
+
 // Brackets a kernel launch with HIP events when timing is enabled
 // (parhip_prof_enable); no-op otherwise.
 class ProfScope {

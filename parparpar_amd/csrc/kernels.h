@@ -111,6 +111,11 @@ hipError_t launch_mac_mfma(const MfmaArgs& a, uint64_t groups, int rt, int nb, i
                            hipStream_t s);
 bool mfma_supported(int rt, int nb, int ib, bool fp4);
 
+// PAR2 repair matrix (gf16_inv.hip): in-place Gauss-Jordan of an n x n
+// block over GF(2^16), one pivot per launch; *bad = first zero pivot or -1.
+hipError_t launch_gf16_invert(uint16_t* a, uint16_t* b, uint32_t n, const uint16_t* exp_tab,
+                              const uint16_t* log_tab, int* bad, uint16_t** result, hipStream_t s);
+
 // PAR3 recovery packets (packets.hip): ws holds 2 x num x chunks x 8 words.
 uint64_t b3_chunks_of(uint64_t block_size);
 hipError_t launch_rec_packets(const uint8_t* rec, uint64_t isid, uint64_t first, uint64_t num,

@@ -86,3 +86,23 @@ class GfProc:
 
 def gf16_coeff(input_num, exponent):
     return lib().parhip_gf16_coeff(input_num, exponent)
+
+
+def rec_matrix_compute(input_valid, recovery):
+    """Galois16RecMatrix::Compute on the device.
+
+    input_valid: booleans per input; recovery: available exponents.  Returns
+    (ok, exponents used, factors) with factors[row, col] = GetFactor(col, row)
+    (valid inputs first, then missing, each in input order); ok is False when
+    too few exponents remain after dropping non-invertible rows."""
+    valid = np.ascontiguousarray(np.asarray(input_valid, dtype=np.uint8))
+    n = int(valid.size)
+    vc = int(valid.sum())
+    rec = np.ascontiguousarray(np.asarray(recovery, dtype=np.uint16)).copy()
+    cnt = ctypes.c_uint32(rec.size)
+    out = np.zeros((n - vc, n), dtype=np.uint16)
+    rc = lib().parhip_gf16_rec_matrix_compute(valid.ctypes.data, n, vc, rec.ctypes.data,
+                                               ctypes.byref(cnt), out.ctypes.data)
+    if rc < 0:
+        check(rc)
+    return rc == 1, rec[:cnt.value].copy(), out
