@@ -45,7 +45,7 @@ template <int NG, int IB>
 __global__ __launch_bounds__(256) void k_gf16_tab(Gf16Args a) {
 	constexpr int TILE = 256 * 4;  // words per work-group
 	__shared__ uint4 tab[IB][NG][4][16];
-	__shared__ uint32_t bas[IB][4 * NG][16];
+	__shared__ __align__(16) uint32_t bas[IB][16][4 * NG];  // [input][j][output pair]
 
 	const int tid = threadIdx.x;
 	const uint64_t w0 = (uint64_t)blockIdx.x * TILE + 4u * tid;
@@ -81,47 +81,47 @@ __global__ __launch_bounds__(256) void k_gf16_tab(Gf16Args a) {
 			d[ib] = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
 		}
 		__syncthreads();  // previous batch finished reading tab / bas
-		// build, phase 1: basis c_e * x^j (j < 16) for every (input, output) of
-		// the batch, output pairs packed in one dword: bas[ib][e/2][j]
-		for (int t = tid; t < IB * 8 * NG; t += 256) {
-			const int m = t % (8 * NG);
-			const int ib = t / (8 * NG);
-			const uint64_t i = ib0 + ib, e = r0 + m;
-			uint32_t c = 0;
-			if (i < i_end && e < a.R) c = a.coeff[e * a.coeff_ld + i];
-			uint16_t* dst = reinterpret_cast<uint16_t*>(&bas[ib][m >> 1][0]) + (m & 1);
+		// build, phase 1: basis c_e * x^j (j < 16) of every (input, output) of
+		// the batch, an output pair per dword: bas[ib][j][pair].  One thread per
+		// (input, pair) writes whole dwords; the lanes of a store cover
+		// consecutive dwords of one row (no bank conflicts).
+		for (int t = tid; t < IB * 4 * NG; t += 256) {
+			const int pr = t % (4 * NG);
+			const int ib = t / (4 * NG);
+			const uint64_t i = ib0 + ib, e0 = r0 + 2 * pr;
+			uint32_t c0 = 0, c1 = 0;
+			if (i < i_end) {
+				if (e0 < a.R) c0 = a.coeff[e0 * a.coeff_ld + i];
+				if (e0 + 1 < a.R) c1 = a.coeff[(e0 + 1) * a.coeff_ld + i];
+			}
 #pragma unroll
 			for (int j = 0; j < 16; j++) {
-				dst[2 * j] = (uint16_t)c;
-				c = gf16_mulx(c);
+				bas[ib][j][pr] = c0 | (c1 << 16);
+				c0 = gf16_mulx(c0);
+				c1 = gf16_mulx(c1);
 			}
 		}
 		__syncthreads();
-		// phase 2: tab[ib][q][k][n] for n in a quarter row, from the 4 basis
-		// dwords of each output pair
-		for (int t = tid; t < IB * NG * 4 * 4; t += 256) {
-			const int quarter = t & 3;
-			const int k = (t >> 2) & 3;
-			const int q = (t >> 4) % NG;
-			const int ib = (t >> 4) / NG;
-			uint4 b[4];  // b[dd] = basis dwords j = 4k..4k+3 of output pair 4q+dd
+		// phase 2: tab[ib][q][k][n] = XOR of the basis rows j = 4k + b for the set
+		// bits b of n, output pairs 4q..4q+3 in the four dwords.  Lane n of each
+		// 16-lane group writes entry n of one 256-byte row: no bank conflicts.
+		for (int t = tid; t < IB * NG * 4 * 16; t += 256) {
+			const unsigned n = t & 15;
+			const int row = t >> 4;
+			const int k = row & 3;
+			const int q = (row >> 2) % NG;
+			const int ib = (row >> 2) / NG;
+			uint4 ent = make_uint4(0, 0, 0, 0);
 #pragma unroll
-			for (int dd = 0; dd < 4; dd++) b[dd] = *reinterpret_cast<const uint4*>(&bas[ib][4 * q + dd][4 * k]);
-#pragma unroll
-			for (int u = 0; u < 4; u++) {
-				const unsigned n = 4 * quarter + u;
-				uint32_t ent[4];
-#pragma unroll
-				for (int dd = 0; dd < 4; dd++) {
-					uint32_t v = 0;
-					if (n & 1) v ^= b[dd].x;
-					if (n & 2) v ^= b[dd].y;
-					if (n & 4) v ^= b[dd].z;
-					if (n & 8) v ^= b[dd].w;
-					ent[dd] = v;
-				}
-				tab[ib][q][k][n] = make_uint4(ent[0], ent[1], ent[2], ent[3]);
+			for (int b = 0; b < 4; b++) {
+				const uint4 v = *reinterpret_cast<const uint4*>(&bas[ib][4 * k + b][4 * q]);
+				const uint32_t m = 0u - ((n >> b) & 1u);
+				ent.x ^= v.x & m;
+				ent.y ^= v.y & m;
+				ent.z ^= v.z & m;
+				ent.w ^= v.w & m;
 			}
+			tab[ib][q][k][n] = ent;
 		}
 		__syncthreads();
 #pragma unroll
