@@ -45,7 +45,9 @@ template <int NG, int IB>
 __global__ __launch_bounds__(256) void k_gf16_tab(Gf16Args a) {
 	constexpr int TILE = 256 * 4;  // words per work-group
 	__shared__ uint4 tab[IB][NG][4][16];
-	__shared__ __align__(16) uint32_t bas[IB][16][4 * NG];  // [input][j][output pair]
+	// [input][j][output pair]; rows padded by 16 B so the 4 position rows a wave
+	// reads together (j = 4k + b, k = 0..3) fall on different banks
+	__shared__ __align__(16) uint32_t bas[IB][16][4 * NG + 4];
 
 	const int tid = threadIdx.x;
 	const uint64_t w0 = (uint64_t)blockIdx.x * TILE + 4u * tid;
