@@ -75,3 +75,40 @@ def test_library_links_no_oracle(built_lib):
         if s.endswith((".py", ".hip", ".cpp", ".h")):
             text = open(s, errors="ignore").read()
             assert "import oracle" not in text and "from oracle" not in text, s
+
+
+@pytest.mark.parametrize("std,compiler", [("c99", "gcc"), ("c++11", "g++")])
+def test_headers_compile_as_c_and_cxx(tmp_path, std, compiler):
+    """Every public header is plain C (for cgo / N-API C callers) and C++."""
+    src = tmp_path / "use_headers.c"
+    lines = ['#include "%s"' % h for h in sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))]
+    lines.append("int main(void) { return PARHIP_OK; }")
+    src.write_text("\n".join(lines) + "\n")
+    cmd = [compiler, "-std=" + std, "-Wall", "-Wextra", "-Werror", "-fsyntax-only"]
+    if compiler == "gcc":
+        cmd.append("-pedantic")
+    else:
+        cmd += ["-x", "c++"]
+    r = subprocess.run(cmd + [str(src)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+
+
+def test_c_program_links_against_library(tmp_path, built_lib):
+    """A C caller links libparhip.so by name and resolves the entry points it uses."""
+    src = tmp_path / "caller.c"
+    src.write_text(
+        '#include "parhip.h"\n#include "parhip_par2.h"\n#include "parhip_batch.h"\n'
+        '#include "parhip_repair.h"\n'
+        "int main(void) {\n"
+        "  void *fns[] = {(void *)parhip_gf64_compute_recovery_blocks,\n"
+        "                 (void *)parhip_par3_create_streaming,\n"
+        "                 (void *)parhip_assemble_recovery_packets,\n"
+        "                 (void *)parhip_gf16_rec_matrix_compute,\n"
+        "                 (void *)parhip_batch_create, (void *)parhip_repair_server_create};\n"
+        "  return fns[0] == 0;\n}\n")
+    exe = tmp_path / "caller"
+    pkg = os.path.join(ROOT, "parparpar_amd")
+    r = subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        "-L", pkg, "-lparhip", "-Wl,-rpath," + pkg, "-Wl,--allow-shlib-undefined"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
