@@ -10,14 +10,20 @@
 //
 // BLAKE3 (published specification; default hash mode) splits the message into
 // 1 KiB chunks, each a sequential chain of up to 16 compressions, then merges
-// the chunk chaining values in a left-complete binary tree.  Three steps:
+// the chunk chaining values in a left-complete binary tree.
 //   k_rec_assemble   headers + block copy (grid-stride, coalesced);
-//   k_b3_chunks      one thread per (packet, chunk); a single-chunk message
-//                    is finished here (ROOT on its last block);
-//   k_b3_tree        one work-group per packet: levels of parent compressions,
-//                    pairs (2t, 2t+1), an odd last node carried up unchanged,
-//                    which is exactly the tree whose left subtree holds the
-//                    largest power of two of chunks; ROOT on the last merge.
+//   k_b3_chunks      a work-group (one wave) = 64 chunks of one packet: the
+//                    64 KiB of message are first copied into LDS with coalesced
+//                    loads (rows padded to 257 dwords, so the lanes' reads fall
+//                    on different banks), then each lane compresses its chunk
+//                    from LDS (the four column steps of a round are independent
+//                    chains), and the group's 64 values are merged in LDS:
+//                    parents (2t, 2t+1) per level, an odd last node carried up.
+//                    An aligned run of 64 chunks is its own subtree of the
+//                    left-complete tree, so the merge is exact; a packet of one
+//                    group (or one chunk) is finished here with ROOT last;
+//   k_b3_tree        merges the per-group values the same way, 1024 per
+//                    work-group per pass, ROOT on the last merge.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -33,6 +39,7 @@ enum : uint32_t { B3_CHUNK_START = 1, B3_CHUNK_END = 2, B3_PARENT = 4, B3_ROOT =
 
 __device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return __builtin_rotateright32(x, n); }
 
+
 #define B3_G(a, b, c, d, mx, my)       \
 	do {                               \
 		v[a] = v[a] + v[b] + (mx);     \
@@ -46,11 +53,10 @@ __device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return __builtin_r
 	} while (0)
 
 // One compression; cv is replaced by the first 8 output words.  The message
-// schedule is applied by index (round r uses m[s_r(i)]) instead of permuting
-// registers, so the words stay in place.
+// schedule is applied by index (round r uses m[S_r(i)], S_r the published
+// permutation applied r times) instead of permuting registers.
 __device__ __forceinline__ void b3_compress(uint32_t cv[8], const uint32_t m[16], uint64_t counter,
                                             uint32_t block_len, uint32_t flags) {
-	// s_r = PERM^r composed: precomputed message index per round and position
 	constexpr uint8_t S[7][16] = {
 	    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15},
 	    {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8},
@@ -139,84 +145,146 @@ __device__ __forceinline__ uint32_t msg_word(const uint8_t* m, uint64_t o, uint6
 	return w;
 }
 
-template <bool ALIGNED>
-__global__ __launch_bounds__(256) void k_b3_chunks(uint8_t* __restrict__ out, uint64_t num,
-                                                   uint64_t bs, uint64_t chunks,
-                                                   uint32_t* __restrict__ cvs) {
-	const uint64_t pkt = 64 + bs, len = 40 + bs;
-	const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= num * chunks) return;
-	const uint64_t p = t / chunks, j = t - p * chunks;
-	const uint8_t* m = out + p * pkt + 24;
-	const uint64_t c0 = j * 1024;
-	const uint64_t clen = (len - c0 < 1024) ? len - c0 : 1024;
-	const uint64_t nblocks = (clen + 63) / 64;  // clen >= 1: the message is >= 41 bytes
-	uint32_t cv[8];
+constexpr int kCvPad = 9;  // LDS row of a chaining value: 8 words + 1 (bank spread)
+
+// Merge n chaining values held in LDS rows (cv, kCvPad words each), a thread
+// per parent, levels separated by barriers; the last merge gets ROOT when
+// `top` and its first 4 words go to digest.  On return the result is cv[0].
+// Every thread of the work-group calls it.
+__device__ __forceinline__ void b3_merge_lds(uint32_t* cv, uint32_t* tmp, uint32_t n, bool top,
+                                             uint32_t* digest) {
+	uint32_t* src = cv;
+	uint32_t* dst = tmp;
+	while (n > 1) {
+		const uint32_t half = n / 2;
+		const bool root = top && n == 2;
+		for (uint32_t t = threadIdx.x; t < half; t += blockDim.x) {
+			uint32_t m[16], h[8];
 #pragma unroll
-	for (int i = 0; i < 8; i++) cv[i] = kB3Iv[i];
-	for (uint64_t b = 0; b < nblocks; b++) {
-		const uint64_t o = c0 + b * 64;
-		uint32_t w[16];
+			for (int i = 0; i < 8; i++) {
+				m[i] = src[(2 * t) * kCvPad + i];
+				m[8 + i] = src[(2 * t + 1) * kCvPad + i];
+				h[i] = kB3Iv[i];
+			}
+			b3_compress(h, m, 0, 64, B3_PARENT | (root ? B3_ROOT : 0u));
+			if (root) {
 #pragma unroll
-		for (int k = 0; k < 16; k++) w[k] = msg_word<ALIGNED>(m, o + 4 * k, len);
-		const uint32_t blen = (uint32_t)((clen - b * 64) < 64 ? clen - b * 64 : 64);
-		uint32_t flags = 0;
-		if (b == 0) flags |= B3_CHUNK_START;
-		if (b == nblocks - 1) flags |= B3_CHUNK_END | (chunks == 1 ? B3_ROOT : 0u);
-		b3_compress(cv, w, j, blen, flags);
+				for (int i = 0; i < 4; i++) digest[i] = h[i];
+			} else {
+#pragma unroll
+				for (int i = 0; i < 8; i++) dst[t * kCvPad + i] = h[i];
+			}
+		}
+		if ((n & 1) && threadIdx.x < 8) dst[half * kCvPad + threadIdx.x] = src[(n - 1) * kCvPad + threadIdx.x];
+		__syncthreads();
+		uint32_t* x = src;
+		src = dst;
+		dst = x;
+		n = half + (n & 1);
 	}
-	if (chunks == 1) {
-		uint8_t* h = out + p * pkt + 8;  // 16-byte truncated digest
+	if (src != cv && threadIdx.x < 8) cv[threadIdx.x] = src[threadIdx.x];
+	__syncthreads();
+}
+
+__device__ __forceinline__ void put_digest(uint8_t* h, const uint32_t* w) {
 #pragma unroll
-		for (int i = 0; i < 4; i++)
+	for (int i = 0; i < 4; i++)
 #pragma unroll
-			for (int b = 0; b < 4; b++) h[4 * i + b] = (uint8_t)(cv[i] >> (8 * b));
-	} else {
+		for (int b = 0; b < 4; b++) h[4 * i + b] = (uint8_t)(w[i] >> (8 * b));
+}
+
+// Work-group = 64 chunks of one packet (blockIdx.x = packet * groups + group):
+// all 256 threads copy the message into LDS, the first wave compresses.
+template <bool ALIGNED>
+__global__ __launch_bounds__(256) void k_b3_chunks(uint8_t* __restrict__ out, uint64_t bs,
+                                                  uint64_t chunks, uint64_t groups,
+                                                  uint32_t* __restrict__ gcv) {
+	constexpr int kRow = 257;  // dwords per chunk row in LDS
+	__shared__ uint32_t msgw[64 * kRow];
+	__shared__ uint32_t cv[64 * kCvPad], tmp[32 * kCvPad];
+	__shared__ uint32_t digest[4];
+	const uint64_t p = blockIdx.x / groups, grp = blockIdx.x - p * groups;
+	const uint64_t pkt = 64 + bs, len = 40 + bs;
+	const uint8_t* msg = out + p * pkt + 24;
+	const uint32_t n = (uint32_t)((chunks - grp * 64) < 64 ? chunks - grp * 64 : 64);
+	// coalesced copy of the group's message words (zero past the end): 16 loads
+	// in flight per thread; a segment that ends inside the message takes plain
+	// aligned loads, the last one the byte-exact path
+	const uint64_t gbase = grp * 65536;
+	const uint32_t total = n * 256;
+	for (uint32_t k0 = 0; k0 < total; k0 += 256 * 16) {
+		uint32_t r[16];
+		if (ALIGNED && gbase + 4ull * (k0 + 256 * 16) <= len) {
 #pragma unroll
-		for (int i = 0; i < 8; i++) cvs[(p * chunks + j) * 8 + i] = cv[i];
+			for (int u = 0; u < 16; u++)
+				r[u] = *reinterpret_cast<const uint32_t*>(msg + gbase + 4ull * (k0 + u * 256 + threadIdx.x));
+		} else {
+#pragma unroll
+			for (int u = 0; u < 16; u++) r[u] = msg_word<ALIGNED>(msg, gbase + 4ull * (k0 + u * 256 + threadIdx.x), len);
+		}
+#pragma unroll
+		for (int u = 0; u < 16; u++) {
+			const uint32_t k = k0 + u * 256 + threadIdx.x;
+			if (k < total) msgw[(k >> 8) * kRow + (k & 255)] = r[u];
+		}
+	}
+	__syncthreads();
+	const uint32_t t = threadIdx.x;
+	if (t < n) {
+		const uint64_t j = grp * 64 + t;
+		const uint64_t c0 = j * 1024;
+		const uint64_t clen = (len - c0 < 1024) ? len - c0 : 1024;
+		const uint32_t nblocks = (uint32_t)((clen + 63) / 64);  // >= 1: the message is >= 41 bytes
+		uint32_t h[8];
+#pragma unroll
+		for (int i = 0; i < 8; i++) h[i] = kB3Iv[i];
+		for (uint32_t b = 0; b < nblocks; b++) {
+			uint32_t w[16];
+#pragma unroll
+			for (int k = 0; k < 16; k++) w[k] = msgw[t * kRow + 16 * b + k];
+			const uint32_t blen = (uint32_t)((clen - b * 64) < 64 ? clen - b * 64 : 64);
+			uint32_t flags = 0;
+			if (b == 0) flags |= B3_CHUNK_START;
+			if (b == nblocks - 1) flags |= B3_CHUNK_END | (chunks == 1 ? B3_ROOT : 0u);
+			b3_compress(h, w, j, blen, flags);
+		}
+		if (chunks == 1) {
+			put_digest(out + p * pkt + 8, h);
+		} else {
+#pragma unroll
+			for (int i = 0; i < 8; i++) cv[t * kCvPad + i] = h[i];
+		}
+	}
+	if (chunks == 1) return;  // uniform
+	__syncthreads();
+	b3_merge_lds(cv, tmp, n, groups == 1, digest);
+	if (groups == 1) {
+		if (threadIdx.x == 0) put_digest(out + p * pkt + 8, digest);
+	} else if (threadIdx.x < 8) {
+		gcv[(p * groups + grp) * 8 + threadIdx.x] = cv[threadIdx.x];
 	}
 }
 
-// One work-group per packet: merge `chunks` chaining values level by level,
-// ping-ponging between a and b (each num x chunks x 8 words).
-__global__ __launch_bounds__(256) void k_b3_tree(uint8_t* __restrict__ out, uint64_t bs,
-                                                 uint64_t chunks, uint32_t* __restrict__ a,
-                                                 uint32_t* __restrict__ b) {
-	const uint64_t p = blockIdx.x;
-	const uint64_t pkt = 64 + bs;
-	uint32_t* src = a + p * chunks * 8;
-	uint32_t* dst = b + p * chunks * 8;
-	uint64_t n = chunks;
-	while (n > 1) {
-		const uint64_t half = n / 2;
-		const bool root = (n == 2);
-		for (uint64_t t = threadIdx.x; t < half; t += blockDim.x) {
-			uint32_t m[16], cv[8];
-#pragma unroll
-			for (int i = 0; i < 16; i++) m[i] = src[2 * t * 8 + i];
-#pragma unroll
-			for (int i = 0; i < 8; i++) cv[i] = kB3Iv[i];
-			b3_compress(cv, m, 0, 64, B3_PARENT | (root ? B3_ROOT : 0u));
-			if (root) {
-				uint8_t* h = out + p * pkt + 8;
-#pragma unroll
-				for (int i = 0; i < 4; i++)
-#pragma unroll
-					for (int k = 0; k < 4; k++) h[4 * i + k] = (uint8_t)(cv[i] >> (8 * k));
-			} else {
-#pragma unroll
-				for (int i = 0; i < 8; i++) dst[t * 8 + i] = cv[i];
-			}
-		}
-		if ((n & 1) && threadIdx.x == 0) {
-#pragma unroll
-			for (int i = 0; i < 8; i++) dst[half * 8 + i] = src[(n - 1) * 8 + i];
-		}
-		__syncthreads();  // this level's nodes are complete before the next reads them
-		uint32_t* tmp = src;
-		src = dst;
-		dst = tmp;
-		n = half + (n & 1);
+// Merge a packet's chaining values in aligned runs of up to 1024 (each an
+// exact subtree: 1024 values of 2^k chunks each): work-group (packet, run)
+// writes the run's value to `outcv`, or the digest when one run is left.
+__global__ __launch_bounds__(256) void k_b3_tree(uint8_t* __restrict__ out, uint64_t bs, uint64_t n_in,
+                                                 const uint32_t* __restrict__ incv, uint64_t runs,
+                                                 uint32_t* __restrict__ outcv) {
+	__shared__ uint32_t cv[1024 * kCvPad];
+	__shared__ uint32_t tmp[512 * kCvPad];
+	__shared__ uint32_t digest[4];
+	const uint64_t p = blockIdx.x / runs, run = blockIdx.x - p * runs;
+	const uint64_t first = run * 1024;
+	const uint32_t n = (uint32_t)((n_in - first) < 1024 ? n_in - first : 1024);
+	for (uint32_t t = threadIdx.x; t < n * 8; t += blockDim.x)
+		cv[(t / 8) * kCvPad + t % 8] = incv[(p * n_in + first) * 8 + t];
+	__syncthreads();
+	b3_merge_lds(cv, tmp, n, runs == 1, digest);
+	if (runs == 1) {
+		if (threadIdx.x == 0) put_digest(out + p * (64 + bs) + 8, digest);
+	} else if (threadIdx.x < 8) {
+		outcv[(p * runs + run) * 8 + threadIdx.x] = cv[threadIdx.x];
 	}
 }
 
@@ -226,6 +294,7 @@ hipError_t launch_rec_packets(const uint8_t* rec, uint64_t isid, uint64_t first,
                               uint64_t bs, uint8_t* out, uint32_t* ws, hipStream_t s) {
 	if (!num) return hipSuccess;
 	const uint64_t chunks = b3_chunks_of(bs);
+	const uint64_t groups = (chunks + 63) / 64;
 	const uintptr_t align = reinterpret_cast<uintptr_t>(rec) | reinterpret_cast<uintptr_t>(out);
 	const int wide = (bs & 7) == 0 && (align & 7) == 0;
 	uint64_t words = wide ? num * bs / 8 : num * bs;
@@ -233,19 +302,26 @@ hipError_t launch_rec_packets(const uint8_t* rec, uint64_t isid, uint64_t first,
 	if (blocks > 16384) blocks = 16384;
 	hipLaunchKernelGGL(k_rec_assemble, dim3((unsigned)blocks), dim3(256), 0, s, rec, isid, first, num,
 	                   bs, out, wide);
-	const uint64_t threads = num * chunks;
-	const uint64_t cblocks = (threads + 255) / 256;
+	const uint64_t cblocks = num * groups;
 	if (cblocks > 0x7fffffffu) return hipErrorInvalidValue;
 	if ((bs & 3) == 0 && (reinterpret_cast<uintptr_t>(out) & 3) == 0)
-		hipLaunchKernelGGL(k_b3_chunks<true>, dim3((unsigned)cblocks), dim3(256), 0, s, out, num, bs,
-		                   chunks, ws);
+		hipLaunchKernelGGL(k_b3_chunks<true>, dim3((unsigned)cblocks), dim3(256), 0, s, out, bs, chunks,
+		                   groups, ws);
 	else
-		hipLaunchKernelGGL(k_b3_chunks<false>, dim3((unsigned)cblocks), dim3(256), 0, s, out, num, bs,
-		                   chunks, ws);
-	if (chunks > 1) {
-		if (num > 0x7fffffffu) return hipErrorInvalidValue;
-		hipLaunchKernelGGL(k_b3_tree, dim3((unsigned)num), dim3(256), 0, s, out, bs, chunks, ws,
-		                   ws + num * chunks * 8);
+		hipLaunchKernelGGL(k_b3_chunks<false>, dim3((unsigned)cblocks), dim3(256), 0, s, out, bs, chunks,
+		                   groups, ws);
+	// ws: num x groups values from k_b3_chunks, then ping-pong halves for the passes
+	uint32_t* a = ws;
+	uint32_t* b = ws + num * groups * 8;
+	for (uint64_t n = groups; n > 1;) {
+		const uint64_t runs = (n + 1023) / 1024;
+		if (num * runs > 0x7fffffffu) return hipErrorInvalidValue;
+		hipLaunchKernelGGL(k_b3_tree, dim3((unsigned)(num * runs)), dim3(256), 0, s, out, bs, n, a, runs,
+		                   b);
+		uint32_t* t = a;
+		a = b;
+		b = t;
+		n = runs == 1 ? 1 : runs;
 	}
 	return hipGetLastError();
 }

@@ -29,7 +29,8 @@ def test_reference_grid(gpu, oracle, num, bs):
         assert got == oracle.assemble_recovery_packets(rec, ISID, first, num, bs), (first,)
 
 
-@pytest.mark.parametrize("bs", [1, 7, 8, 12, 983, 984, 985, 1023, 1024, 2047, 3000, 4097])
+@pytest.mark.parametrize("bs", [1, 7, 8, 12, 983, 984, 985, 1023, 1024, 2047, 3000, 4097,
+                                64 * 1024 - 40, 64 * 1024 - 39, 65 * 1024 - 40, 131032])
 def test_odd_and_edge_block_sizes(gpu, oracle, bs):
     from parparpar_amd import binding
     rng = np.random.default_rng(bs)
@@ -100,3 +101,13 @@ def test_argument_errors(gpu):
     assert P.lib().parhip_assemble_recovery_packets(b"\0" * 8, ISID, 0, 0, 8, bytes(out)) == P.PARHIP_ERR_RANGE
     assert P.lib().parhip_assemble_recovery_packets(b"\0" * 8, ISID, 0, 1, -8, bytes(out)) == P.PARHIP_ERR_RANGE
     assert P.lib().parhip_assemble_recovery_packets(b"\0" * 8, ISID, 0, 1 << 40, 2, bytes(out)) == P.PARHIP_ERR_RANGE
+
+
+def test_block_above_64_mib(gpu, oracle):
+    """More than 1024 groups of 64 chunks: the parent tree takes two passes."""
+    from parparpar_amd import binding
+    bs = 65 * 1024 * 1024 + 8
+    rng = np.random.default_rng(65)
+    rec = rng.bytes(2 * bs)
+    got = binding.assemble_recovery_packets(rec, ISID, 5, 2, bs)
+    assert got == oracle.assemble_recovery_packets(rec, ISID, 5, 2, bs)
