@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--slices", type=int, default=1024)
     ap.add_argument("--slice-bytes", type=int, default=1 << 20)
     ap.add_argument("--recovery", type=int, default=8)
-    ap.add_argument("--method", default="auto", choices=["auto", "table", "mfma", "mfma-i8"],
+    ap.add_argument("--method", default="auto", choices=["auto", "table", "table8", "mfma", "mfma-i8"],
                     help="contraction strategy: nibble tables in LDS, or the GF(2) bit-matrix "
                          "on fp4 / int8 MFMA")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
@@ -152,10 +152,10 @@ def main_par3(args):
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     L = P.lib()
-    P.set_method({"auto": P.METHOD_AUTO, "table": P.METHOD_TABLE, "mfma": P.METHOD_MFMA,
-                  "mfma-i8": P.METHOD_MFMA_I8}[args.method])
+    P.set_method({"auto": P.METHOD_AUTO, "table": P.METHOD_TABLE, "table8": P.METHOD_TABLE8,
+                  "mfma": P.METHOD_MFMA, "mfma-i8": P.METHOD_MFMA_I8}[args.method])
     mfma = args.method in ("mfma", "mfma-i8")
-    mac_name = "mac_mfma" if mfma else "mac_tab"
+    mac_name = {"mfma": "mac_mfma", "mfma-i8": "mac_mfma", "table8": "mac_tab8"}.get(args.method, "mac_tab")
 
     def step():
         P.check(L.parhip_d_gf64_build_cauchy(d_coeff.data_ptr(), n, r, 0, n, sp))
@@ -207,7 +207,7 @@ def main_par3(args):
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_mac_tab_1024x1MiB_r8.txt")
-    if os.path.exists(tpath) and not mfma and (n, sb, r) == (1024, 1 << 20, 8):
+    if os.path.exists(tpath) and args.method in ("auto", "table") and (n, sb, r) == (1024, 1 << 20, 8):
         for line in open(tpath):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = int(line.split(":")[1].strip())
@@ -229,7 +229,8 @@ def main_par3(args):
             "workload": "PAR3 create 1 GiB = %d slices x %d KiB, R=%d, GF(2^64) Cauchy points (0, %d)"
                         % (n, sb // 1024, r, n),
             "slices": n, "slice_bytes": sb, "recovery": r, "parallelism": "single GPU",
-            "method": {"mfma": "GF(2) bit-matrix on fp4 MFMA (k_mac_mfma)",
+            "method": {"table8": "8-bit split tables in LDS (k_mac_tab8) + split-K XOR reduce",
+                       "mfma": "GF(2) bit-matrix on fp4 MFMA (k_mac_mfma)",
                        "mfma-i8": "GF(2) bit-matrix on int8 MFMA, int32 parity (k_mac_mfma)"}.get(
                 args.method, "nibble split tables in LDS (k_mac_tab) + split-K XOR reduce"),
         },

@@ -46,6 +46,7 @@ extern "C" {
 #define PARHIP_METHOD_TABLE 1    /* split nibble tables in LDS                     */
 #define PARHIP_METHOD_MFMA 2     /* GF(2) bit-matrix on fp4 MFMA (exact 0/1 counts) */
 #define PARHIP_METHOD_MFMA_I8 3  /* GF(2) bit-matrix on int8 MFMA (int32 parity)    */
+#define PARHIP_METHOD_TABLE8 4   /* 8-bit split tables in LDS (256-entry rows)      */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int parhip_version(void);
@@ -54,7 +55,8 @@ const char *parhip_last_error(void);
 /* Select the HIP device used by calls from this thread (default 0). */
 int parhip_set_device(int device);
 /* Force a contraction strategy for this thread (PARHIP_METHOD_*). The MFMA
- * strategies apply to strided slices; pointer-array calls use the tables. */
+ * and 8-bit table strategies apply to strided slices; pointer-array calls use
+ * the nibble tables. */
 int parhip_set_method(int method);
 /* Force the table-contraction launch shape for this thread (tuning; no
  * counterpart in the reference). rt = outputs per work-group, wpt = words per

@@ -116,6 +116,7 @@ METHOD_AUTO = 0
 METHOD_TABLE = 1
 METHOD_MFMA = 2
 METHOD_MFMA_I8 = 3
+METHOD_TABLE8 = 4
 
 
 class ParhipError(RuntimeError):

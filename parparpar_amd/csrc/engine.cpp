@@ -383,6 +383,50 @@ static int run_contraction_mfma(const MacArgs& m, bool fp4, hipStream_t s) {
 	return e == hipSuccess ? PARHIP_OK : hip_fail(e, "k_mac_mfma");
 }
 
+// 8-bit split tables (gf64_tab8.hip): 4-output tiles for 3..8 outputs, else
+// 2 (one input per LDS batch: 32 or 64 KiB of tables), 8 words per thread on
+// wide slices so that a table is read 8x per build, split-K to fill the device.
+static int run_contraction_tab8(MacArgs a, hipStream_t s) {
+	int rt = (a.R > 2 && a.R <= 8) ? 4 : 2, wpt = a.W >= 2048 ? 8 : (a.W >= 1024 ? 4 : 2), ib = 1;
+	if ((t_tune_rt || t_tune_wpt || t_tune_ib) &&
+	    mac_tab8_supported(t_tune_rt ? t_tune_rt : rt, t_tune_wpt ? t_tune_wpt : wpt,
+	                       t_tune_ib ? t_tune_ib : ib)) {
+		rt = t_tune_rt ? t_tune_rt : rt;
+		wpt = t_tune_wpt ? t_tune_wpt : wpt;
+		ib = t_tune_ib ? t_tune_ib : ib;
+	}
+	const uint64_t base = ((a.W + 256ull * wpt - 1) / (256ull * wpt)) * ((a.R + rt - 1) / rt);
+	const uint64_t per_cu = std::max<uint64_t>(1, 160 / ((uint64_t)ib * rt / 2 * 32));
+	const uint64_t target = (uint64_t)device_cus(t_device) * per_cu;
+	uint64_t groups = base < target ? (target + base - 1) / base : 1;
+	if (t_tune_groups > 0) groups = (uint64_t)t_tune_groups;
+	groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, a.N / (2 * ib)));
+	groups = std::min<uint64_t>(groups, 65535);
+	uint64_t per = (a.N + groups - 1) / groups;
+	per = (per + ib - 1) / ib * ib;
+	groups = (a.N + per - 1) / per;
+	a.in_per_group = per;
+	a.partial = nullptr;
+	void* ws = nullptr;
+	if (groups > 1) {
+		int rc = dev_alloc(&ws, (size_t)xor_reduce_scratch_words(groups, a.R, a.W) * sizeof(uint64_t), s);
+		if (rc) return rc;
+		a.partial = (uint64_t*)ws;
+	}
+	hipError_t e;
+	{
+		ProfScope ps("mac_tab8", s);
+		e = launch_mac_tab8(a, groups, rt, wpt, ib, s);
+	}
+	if (e == hipSuccess && groups > 1) {
+		ProfScope ps("xor_reduce", s);
+		e = launch_xor_reduce(a.partial, groups, a.R, a.W, a.out_base, a.out_stride, nullptr,
+		                      a.accumulate, s);
+	}
+	dev_free(ws, s);
+	return e == hipSuccess ? PARHIP_OK : hip_fail(e, "k_mac_tab8");
+}
+
 // Enqueue the full contraction described by `a` (partial/in_per_group are
 // filled in here) on stream s.
 int run_contraction(MacArgs a, hipStream_t s) {
@@ -391,6 +435,7 @@ int run_contraction(MacArgs a, hipStream_t s) {
 	if ((t_method == PARHIP_METHOD_MFMA || t_method == PARHIP_METHOD_MFMA_I8) && !a.in_ptrs &&
 	    !a.out_ptrs)
 		return run_contraction_mfma(a, t_method == PARHIP_METHOD_MFMA, s);
+	if (t_method == PARHIP_METHOD_TABLE8 && !a.in_ptrs && !a.out_ptrs) return run_contraction_tab8(a, s);
 	Plan p = plan_contraction(a.N, a.R, a.W, t_device);
 	a.in_per_group = p.in_per_group;
 	a.partial = nullptr;
@@ -439,7 +484,7 @@ int parhip_set_device(int device) {
 }
 
 int parhip_set_method(int method) {
-	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_MFMA_I8)
+	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_TABLE8)
 		return fail(PARHIP_ERR_ARG, "parhip_set_method: unknown method %d", method);
 	t_method = method;
 	return PARHIP_OK;
@@ -467,6 +512,7 @@ int parhip_prof_get(const char* kernel, double* total_ms, uint64_t* launches) {
 
 int parhip_tune_override(int rt, int wpt, int ib, int64_t groups) {
 	if ((rt || wpt || ib) && !mac_tab_supported(rt ? rt : 8, wpt ? wpt : 4, ib ? ib : 4) &&
+	    !mac_tab8_supported(rt ? rt : 4, wpt ? wpt : 4, ib ? ib : 1) &&
 	    !mfma_supported(rt ? rt : 4, wpt ? wpt : 2, ib ? ib : 2, true) &&
 	    !mfma_supported(rt ? rt : 4, wpt ? wpt : 2, ib ? ib : 2, false))
 		return fail(PARHIP_ERR_ARG, "tune_override: unsupported (rt=%d wpt=%d ib=%d)", rt, wpt, ib);

@@ -26,6 +26,12 @@ __host__ __device__ __forceinline__ uint64_t gf64_mulx_pow(uint64_t c, unsigned 
 	return gf64_reduce(c << j, c >> (64u - j));
 }
 
+// c * x^j mod p for 0 <= j < 64 without a branch on j (the high part is
+// shifted in two steps so that j = 0 gives 0 instead of an undefined shift).
+__host__ __device__ __forceinline__ uint64_t gf64_mulx_pow_any(uint64_t c, unsigned j) {
+	return gf64_reduce(c << j, (c >> 1) >> (63u - j));
+}
+
 // c * x^4: the 4 bits shifted out fold back as h * (x^4 + x^3 + x + 1), at
 // most 8 bits, so one fold suffices.
 __host__ __device__ __forceinline__ uint64_t gf64_mulx4(uint64_t c) {

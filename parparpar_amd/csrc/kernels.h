@@ -105,6 +105,11 @@ struct MfmaArgs {
 	uint64_t* partial;
 	int accumulate;
 };
+// 8-bit split tables (gf64_tab8.hip): RT in {2, 4}, WPT in {2, 4, 8}, IB in {1, 2}
+// (IB = 1 with WPT = 8).
+bool mac_tab8_supported(int rt, int wpt, int ib);
+hipError_t launch_mac_tab8(const MacArgs& a, uint64_t groups, int rt, int wpt, int ib, hipStream_t s);
+
 hipError_t launch_gf64_rows(const uint64_t* coeff, uint64_t ld, uint64_t N, uint64_t R,
                             uint64_t* rows, hipStream_t s);
 hipError_t launch_mac_mfma(const MfmaArgs& a, uint64_t groups, int rt, int nb, int ib, bool fp4,

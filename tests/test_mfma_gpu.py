@@ -1,21 +1,22 @@
-"""GF(2) bit-matrix strategy on MFMA (fp4 and int8) against the oracle.
+"""The alternative contraction strategies against the oracle: GF(2) bit-matrix
+on MFMA (fp4, int8) and 8-bit split tables.
 
-The same contraction as the table kernel, computed as an integer matrix
-product over bits whose parity is the GF(2^64) result.  Bit-exact against the
-oracle on small shapes and on sampled columns of a 1 GiB create, and equal to
-the table strategy's output.
+The same contraction as the nibble-table kernel: as an integer matrix product
+over bits whose parity is the GF(2^64) result, or with byte tables.  Bit-exact
+against the oracle on small shapes and on sampled columns of a 1 GiB create,
+and equal to the nibble-table strategy's output.
 """
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-METHODS = ["fp4", "i8"]
+METHODS = ["fp4", "i8", "tab8"]
 
 
 def _method(name):
     import parparpar_amd as pp
-    return pp.METHOD_MFMA if name == "fp4" else pp.METHOD_MFMA_I8
+    return {"fp4": pp.METHOD_MFMA, "i8": pp.METHOD_MFMA_I8, "tab8": pp.METHOD_TABLE8}[name]
 
 
 @pytest.fixture
