@@ -85,6 +85,33 @@ int parhip_gf64_compute_recovery_blocks_full(const uint64_t *inputs, size_t numI
                                              uint64_t firstRecovery, int numThreads,
                                              int accumulate);
 
+/* GF64Controller::ComputeRecoveryBlocksBarycentric (src/par3_engine.h:209-217,
+ * src/par3_engine_barycentric.cc:104-280; N-API compute_recovery_barycentric,
+ * src/gf64_addon.cc:1636-1796).  For R <= 32 the reference delegates to
+ * ComputeRecoveryBlocks; for R > 32 it inverts (firstInput+c) XOR
+ * (firstRecovery+r) (0 -> 1) per row and accumulates, which is the same
+ * Cauchy product: the values equal parhip_gf64_compute_recovery_blocks. */
+int parhip_gf64_compute_recovery_blocks_barycentric(const uint64_t *inputs, size_t numInputs,
+                                                    uint64_t *recovery, size_t numRecovery,
+                                                    size_t blockSize64, uint64_t firstInput,
+                                                    uint64_t firstRecovery, int numThreads,
+                                                    int accumulate);
+
+/* N-API compute_recovery_streaming (src/gf64_addon.cc:2349-2650): the input
+ * slices arrive as numBatches host buffers, batch b holding batchBlocks[b]
+ * whole slices; concatenated in order they are the numInputs inputs (global
+ * indices firstInput, firstInput+1, ...).  The addon copies them into one
+ * buffer and calls the engine once (Fenger, Barycentric or Cauchy: the same
+ * values); here they are streamed to the device from where they lie.
+ * Recovery is written, not accumulated.  numInputs == 0 or numRecovery == 0
+ * returns without touching recovery; batch counts that do not sum to
+ * numInputs give PARHIP_ERR_RANGE. */
+int parhip_gf64_compute_recovery_batches(const uint64_t *const *batches, const size_t *batchBlocks,
+                                         size_t numBatches, uint64_t *recovery, size_t numInputs,
+                                         size_t numRecovery, size_t blockSize64,
+                                         uint64_t firstInput, uint64_t firstRecovery,
+                                         int numThreads);
+
 /* GF64Controller::ComputeRecoveryBlocksWithCoeff (src/par3_engine.h:65-72,
  * src/par3_engine.cc:1165-1382); coeff is numRecovery x numInputs row-major. */
 int parhip_gf64_compute_recovery_blocks_with_coeff(const uint64_t *inputs, size_t numInputs,

@@ -9,6 +9,10 @@ reference's own call sequences.  Every call goes to libparhip.so.
                      firstInput, firstRecovery[, numThreads])       gf64_addon.cc:1266
     compute_recovery_full(...same...)                               gf64_addon.cc:1443
     compute_recovery_accumulate(...same...)  (accumulate=true)      gf64_addon.cc:1799
+    compute_recovery_barycentric(...same...[, accumulate])          gf64_addon.cc:1636
+    compute_recovery_streaming(inputBatches, outputBuffer, numInputs, numRecovery,
+                     blockSize, firstInput, firstRecovery[, numThreads])
+                                                                    gf64_addon.cc:2349
     compute_recovery_with_coeff(coeffMatrix, inputs, outputs, numInputs,
                      numRecovery, blockSize, firstInput, firstRecovery[, numThreads])
                                                                     gf64_addon.cc:2189
@@ -76,6 +80,61 @@ def compute_recovery_accumulate(inputs, outputs, numInputs, numRecovery, blockSi
                                 firstRecovery, numThreads=0):
     return compute_recovery_full(inputs, outputs, numInputs, numRecovery, blockSize, firstInput,
                                  firstRecovery, numThreads, accumulate=True)
+
+
+def compute_recovery_barycentric(inputs, outputs, numInputs, numRecovery, blockSize, firstInput,
+                                 firstRecovery, numThreads=0, accumulate=False):
+    """Counts must be positive, as the addon requires (gf64_addon.cc:1746-1755)."""
+    if numInputs <= 0:
+        raise ValueError("numInputs must be positive")
+    if numRecovery <= 0:
+        raise ValueError("numRecovery must be positive")
+    _check_block(blockSize)
+    _check_sizes(inputs, numInputs, blockSize, "inputs")
+    _check_sizes(outputs, numRecovery, blockSize, "outputs")
+    check(lib().parhip_gf64_compute_recovery_blocks_barycentric(
+        _buf(inputs, "inputs"), numInputs, _buf(outputs, "outputs"), numRecovery, blockSize // 8,
+        firstInput, firstRecovery, numThreads, 1 if accumulate else 0))
+    return outputs
+
+
+def compute_recovery_streaming(inputBatches, outputBuffer, numInputs, numRecovery, blockSize,
+                               firstInput, firstRecovery, numThreads=0):
+    """All input batches in one call; the argument rules of gf64_addon.cc:2349-2525
+    (TypeError for a non-list or a non-buffer element, RangeError -> ValueError for
+    negative counts, a bad blockSize, a short output, a batch that is not whole
+    slices, or batches that do not hold exactly numInputs slices)."""
+    if not isinstance(inputBatches, (list, tuple)):
+        raise TypeError("inputBatches must be an array of Buffers")
+    if numInputs < 0:
+        raise ValueError("numInputs must be non-negative")
+    if numRecovery < 0:
+        raise ValueError("numRecovery must be non-negative")
+    _check_block(blockSize)
+    if numInputs == 0 or numRecovery == 0:
+        return outputBuffer
+    _check_sizes(outputBuffer, numRecovery, blockSize, "outputBuffer")
+    counts, total = [], 0
+    for b in inputBatches:
+        if not isinstance(b, np.ndarray):
+            raise TypeError("inputBatches element must be a Buffer")
+        if b.nbytes % blockSize:
+            raise ValueError("inputBatches element length not a multiple of blockSize")
+        k = b.nbytes // blockSize
+        if total + k > numInputs:
+            raise ValueError("inputBatches total blocks exceeds numInputs")
+        _buf(b, "inputBatches element")
+        counts.append(k)
+        total += k
+    if total != numInputs:
+        raise ValueError("inputBatches total blocks does not equal numInputs")
+    n = len(inputBatches)
+    ptrs = (ctypes.c_void_p * max(n, 1))(*[b.ctypes.data for b in inputBatches])
+    cnt = (ctypes.c_size_t * max(n, 1))(*counts)
+    check(lib().parhip_gf64_compute_recovery_batches(
+        ptrs, cnt, n, _buf(outputBuffer, "outputBuffer"), numInputs, numRecovery, blockSize // 8,
+        firstInput, firstRecovery, numThreads))
+    return outputBuffer
 
 
 def compute_recovery_with_coeff(coeffMatrix, inputs, outputs, numInputs, numRecovery, blockSize,

@@ -129,15 +129,39 @@ static int stream_contract(const FillFn& fill, uint64_t N, uint64_t W, uint64_t*
 	return stream_slices(N, (size_t)W * sizeof(uint64_t), fill, consume, s);
 }
 
+// memcpy split into 4 MiB parts over the copy threads.
+static void parallel_copy(void* dst, const void* src, size_t bytes) {
+	const size_t part = 4u << 20;
+	parallel_parts((bytes + part - 1) / part, [&](size_t p) {
+		const size_t o = p * part;
+		memcpy((char*)dst + o, (const char*)src + o, std::min(part, bytes - o));
+	});
+}
+
 // Parallel copy of `count` slices from a contiguous host buffer into a pinned slot.
 static FillFn memcpy_fill(const uint64_t* inputs, size_t slice_bytes) {
 	return [inputs, slice_bytes](void* dst, uint64_t first, uint64_t count) {
-		const char* src = (const char*)inputs + first * slice_bytes;
-		const size_t bytes = (size_t)count * slice_bytes, part = 4u << 20;
-		parallel_parts((bytes + part - 1) / part, [&](size_t p) {
-			const size_t o = p * part;
-			memcpy((char*)dst + o, src + o, std::min(part, bytes - o));
-		});
+		parallel_copy(dst, (const char*)inputs + first * slice_bytes, (size_t)count * slice_bytes);
+		return PARHIP_OK;
+	};
+}
+
+// Slices [first, first + count) gathered from a list of host batches whose
+// running slice offsets are `starts` (starts[b] = first slice of batch b,
+// starts.back() = total); a chunk may span several batches.
+static FillFn batches_fill(const uint64_t* const* batches, const std::vector<uint64_t>& starts,
+                           size_t slice_bytes) {
+	return [batches, &starts, slice_bytes](void* dst, uint64_t first, uint64_t count) {
+		const uint64_t end = first + count;
+		size_t b = std::upper_bound(starts.begin(), starts.end(), first) - starts.begin() - 1;
+		for (uint64_t j = first; j < end; b++) {
+			const uint64_t take = std::min(end, starts[b + 1]) - j;
+			if (take == 0) continue;  // empty batch
+			parallel_copy((char*)dst + (j - first) * slice_bytes,
+			              (const char*)batches[b] + (j - starts[b]) * slice_bytes,
+			              (size_t)take * slice_bytes);
+			j += take;
+		}
 		return PARHIP_OK;
 	};
 }
@@ -227,6 +251,70 @@ int parhip_gf64_compute_recovery_blocks(const uint64_t* inputs, size_t numInputs
 	RC(parhip_d_gf64_recovery(din.as<uint64_t>(), numInputs, blockSize64, dout.as<uint64_t>(),
 	                          numRecovery, blockSize64, blockSize64, firstInput, firstRecovery,
 	                          accumulate, s));
+	RC(d2h(recovery, dout.p, out_bytes, s));
+	return sync(s);
+}
+
+int parhip_gf64_compute_recovery_blocks_barycentric(const uint64_t* inputs, size_t numInputs,
+                                                    uint64_t* recovery, size_t numRecovery,
+                                                    size_t blockSize64, uint64_t firstInput,
+                                                    uint64_t firstRecovery, int numThreads,
+                                                    int accumulate) {
+	// Both of the reference's branches compute the Cauchy product
+	// (src/par3_engine_barycentric.cc:121-128 delegate, :170-229 per-row
+	// inverse + muladd_arr).
+	return parhip_gf64_compute_recovery_blocks(inputs, numInputs, recovery, numRecovery,
+	                                           blockSize64, firstInput, firstRecovery, numThreads,
+	                                           accumulate);
+}
+
+int parhip_gf64_compute_recovery_batches(const uint64_t* const* batches, const size_t* batchBlocks,
+                                         size_t numBatches, uint64_t* recovery, size_t numInputs,
+                                         size_t numRecovery, size_t blockSize64,
+                                         uint64_t firstInput, uint64_t firstRecovery,
+                                         int numThreads) {
+	(void)numThreads;
+	if (numInputs == 0 || numRecovery == 0) return PARHIP_OK;
+	if (!recovery || (numBatches && (!batches || !batchBlocks)))
+		return fail(PARHIP_ERR_ARG, "compute_recovery_batches: null pointer");
+	std::vector<uint64_t> starts(numBatches + 1, 0);
+	for (size_t b = 0; b < numBatches; b++) {
+		if (batchBlocks[b] && !batches[b])
+			return fail(PARHIP_ERR_ARG, "compute_recovery_batches: batch %zu is null", b);
+		if (batchBlocks[b] > numInputs - starts[b])
+			return fail(PARHIP_ERR_RANGE, "compute_recovery_batches: batches hold more than %zu slices",
+			            numInputs);
+		starts[b + 1] = starts[b] + batchBlocks[b];
+	}
+	if (starts[numBatches] != numInputs)
+		return fail(PARHIP_ERR_RANGE, "compute_recovery_batches: batches hold %llu slices, numInputs is %zu",
+		            (unsigned long long)starts[numBatches], numInputs);
+	if (blockSize64 == 0) return PARHIP_OK;
+	size_t in_bytes, out_bytes, cb;
+	RC(words_bytes(numInputs, blockSize64, &in_bytes));
+	RC(words_bytes(numRecovery, blockSize64, &out_bytes));
+	RC(words_bytes(numRecovery, numInputs, &cb));
+	hipStream_t s;
+	RC(host_stream(&s));
+	const size_t slice_bytes = blockSize64 * 8;
+	DevBuf dout, dco;
+	RC(dout.alloc(out_bytes, s));
+	RC(dco.alloc(cb, s));
+	RC(parhip_d_gf64_build_cauchy(dco.as<uint64_t>(), numInputs, numRecovery, firstInput,
+	                              firstRecovery, s));
+	if (use_streaming(in_bytes)) {
+		RC(stream_contract(batches_fill(batches, starts, slice_bytes), numInputs, blockSize64,
+		                   dout.as<uint64_t>(), numRecovery, dco.as<uint64_t>(), numInputs, 0, s));
+	} else {
+		DevBuf din;
+		RC(din.alloc(in_bytes, s));
+		for (size_t b = 0; b < numBatches; b++)
+			if (batchBlocks[b])
+				RC(h2d(din.as<char>() + starts[b] * slice_bytes, batches[b], batchBlocks[b] * slice_bytes, s));
+		RC(parhip_d_gf64_recovery_with_coeff(din.as<uint64_t>(), numInputs, blockSize64,
+		                                     dout.as<uint64_t>(), numRecovery, blockSize64,
+		                                     blockSize64, dco.as<uint64_t>(), 0, s));
+	}
 	RC(d2h(recovery, dout.p, out_bytes, s));
 	return sync(s);
 }

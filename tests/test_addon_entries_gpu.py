@@ -1,0 +1,114 @@
+"""The addon's barycentric and streaming recovery entries against the oracle.
+
+compute_recovery_barycentric (src/gf64_addon.cc:1636-1796) routes to
+GF64Controller::ComputeRecoveryBlocksBarycentric, whose two branches (R <= 32
+delegation, R > 32 per-row inverse + muladd_arr,
+src/par3_engine_barycentric.cc:121-229) both compute the Cauchy product;
+compute_recovery_streaming (src/gf64_addon.cc:2349-2650) concatenates a list of
+input batches and computes the same product once.  Cases: R on both sides of
+32, accumulate, uneven and empty batches, batches that straddle staging chunks,
+and the argument errors of both bindings.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_words(rng, *shape):
+    return rng.integers(0, 2 ** 64, size=shape, dtype=np.uint64)
+
+
+@pytest.mark.parametrize("n,w,r,fi,fr", [(5, 128, 2, 0, 5), (40, 64, 33, 0, 40), (17, 96, 48, 100, 3),
+                                         (64, 512, 8, 0, 64)])
+def test_barycentric(gpu, oracle, n, w, r, fi, fr):
+    from parparpar_amd import binding
+    rng = np.random.default_rng(n * r)
+    inp = rand_words(rng, n, w)
+    out = np.zeros((r, w), dtype=np.uint64)
+    binding.compute_recovery_barycentric(inp, out, n, r, w * 8, fi, fr)
+    want = oracle.recovery(inp, r, fi, fr)
+    assert np.array_equal(out, want)
+    prev = rand_words(rng, r, w)
+    acc = prev.copy()
+    binding.compute_recovery_barycentric(inp, acc, n, r, w * 8, fi, fr, 0, True)
+    assert np.array_equal(acc, prev ^ want)
+
+
+def test_barycentric_errors(gpu):
+    from parparpar_amd import binding
+    buf = np.zeros(16, dtype=np.uint64)
+    with pytest.raises(ValueError):
+        binding.compute_recovery_barycentric(buf, buf, 0, 1, 8, 0, 1)
+    with pytest.raises(ValueError):
+        binding.compute_recovery_barycentric(buf, buf, 1, 0, 8, 0, 1)
+    with pytest.raises(ValueError):
+        binding.compute_recovery_barycentric(buf, buf, 1, 1, 12, 0, 1)
+    with pytest.raises(ValueError):
+        binding.compute_recovery_barycentric(buf, buf, 3, 1, 64, 0, 3)
+
+
+@pytest.mark.parametrize("sizes", [[7], [1, 2, 3, 4], [0, 5, 0, 0, 6, 0], [10, 0]])
+def test_streaming_batches(gpu, oracle, sizes):
+    from parparpar_amd import binding
+    rng = np.random.default_rng(sum(sizes) * 13 + len(sizes))
+    n, w, r = sum(sizes), 80, 5
+    inp = rand_words(rng, n, w)
+    batches, o = [], 0
+    for k in sizes:
+        batches.append(inp[o:o + k].copy())
+        o += k
+    out = np.full((r, w), 0xAB, dtype=np.uint64)  # written, not accumulated
+    binding.compute_recovery_streaming(batches, out, n, r, w * 8, 3, n + 3)
+    assert np.array_equal(out, oracle.recovery(inp, r, 3, n + 3))
+
+
+def test_streaming_through_the_staging_ring(gpu, oracle):
+    """Small staging chunks (3 slices) so chunks start and end inside batches."""
+    from parparpar_amd import binding
+    rng = np.random.default_rng(77)
+    sizes = [2, 5, 0, 1, 7, 4]
+    n, w, r = sum(sizes), 256, 4
+    inp = rand_words(rng, n, w)
+    batches, o = [], 0
+    for k in sizes:
+        batches.append(inp[o:o + k].copy())
+        o += k
+    binding.set_stream_chunk(3 * w * 8)
+    try:
+        out = np.zeros((r, w), dtype=np.uint64)
+        binding.compute_recovery_streaming(batches, out, n, r, w * 8, 0, n)
+    finally:
+        binding.set_stream_chunk(0)
+    assert np.array_equal(out, oracle.recovery(inp, r, 0, n))
+
+
+def test_streaming_errors(gpu):
+    import parparpar_amd as P
+    from parparpar_amd import binding
+    out = np.zeros((2, 8), dtype=np.uint64)
+    b = np.zeros((3, 8), dtype=np.uint64)
+    with pytest.raises(TypeError):
+        binding.compute_recovery_streaming(b, out, 3, 2, 64, 0, 3)
+    with pytest.raises(TypeError):
+        binding.compute_recovery_streaming([b, b"x" * 64], out, 4, 2, 64, 0, 4)
+    with pytest.raises(ValueError):
+        binding.compute_recovery_streaming([b], out, -1, 2, 64, 0, 3)
+    with pytest.raises(ValueError):
+        binding.compute_recovery_streaming([b], out, 3, 2, 60, 0, 3)
+    with pytest.raises(ValueError):
+        binding.compute_recovery_streaming([b], out, 2, 2, 64, 0, 3)  # exceeds numInputs
+    with pytest.raises(ValueError):
+        binding.compute_recovery_streaming([b], out, 4, 2, 64, 0, 4)  # does not equal numInputs
+    with pytest.raises(ValueError):
+        binding.compute_recovery_streaming([np.zeros(9, dtype=np.uint8)], out, 1, 2, 8, 0, 1)
+    # zero counts: outputs untouched
+    keep = np.full((2, 8), 7, dtype=np.uint64)
+    binding.compute_recovery_streaming([], keep, 0, 2, 64, 0, 0)
+    assert (keep == 7).all()
+    # the C entry checks the batch counts itself
+    import ctypes
+    ptrs = (ctypes.c_void_p * 1)(b.ctypes.data)
+    cnt = (ctypes.c_size_t * 1)(2)
+    assert P.lib().parhip_gf64_compute_recovery_batches(ptrs, cnt, 1, out.ctypes.data, 3, 2, 8, 0, 3,
+                                                        0) == P.PARHIP_ERR_RANGE
