@@ -97,6 +97,17 @@ int parhip_gf64_compute_recovery_blocks_barycentric(const uint64_t *inputs, size
                                                     uint64_t firstRecovery, int numThreads,
                                                     int accumulate);
 
+/* GF64Controller::ComputeRecoveryBlocksFenger (src/par3_engine.h:237-244,
+ * src/par3_engine_fenger.cc:109-296;
+ * N-API compute_recovery_fenger, src/gf64_addon.cc:1980-2078).  The reference
+ * documents it as bit-exact with ComputeRecoveryBlocks for any workload (it is
+ * an asymptotically fast evaluation of the same Cauchy product, chosen there
+ * for R >= 8192); here it is that contraction.  Recovery is written. */
+int parhip_gf64_compute_recovery_blocks_fenger(const uint64_t *inputs, size_t numInputs,
+                                               uint64_t *recovery, size_t numRecovery,
+                                               size_t blockSize64, uint64_t firstInput,
+                                               uint64_t firstRecovery, int numThreads);
+
 /* N-API compute_recovery_streaming (src/gf64_addon.cc:2349-2650): the input
  * slices arrive as numBatches host buffers, batch b holding batchBlocks[b]
  * whole slices; concatenated in order they are the numInputs inputs (global

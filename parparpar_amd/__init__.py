@@ -49,6 +49,7 @@ _SIGS = [
     ("parhip_gf64_compute_recovery_blocks_full", _int, [_vp, _sz, _vp, _sz, _sz, _u64, _u64, _int, _int]),
     ("parhip_gf64_compute_recovery_blocks_barycentric", _int,
      [_vp, _sz, _vp, _sz, _sz, _u64, _u64, _int, _int]),
+    ("parhip_gf64_compute_recovery_blocks_fenger", _int, [_vp, _sz, _vp, _sz, _sz, _u64, _u64, _int]),
     ("parhip_gf64_compute_recovery_batches", _int,
      [_vp, _vp, _sz, _vp, _sz, _sz, _sz, _u64, _u64, _int]),
     ("parhip_gf64_compute_recovery_blocks_with_coeff", _int, [_vp, _sz, _vp, _sz, _sz, _vp, _int, _int]),

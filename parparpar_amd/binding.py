@@ -10,6 +10,7 @@ reference's own call sequences.  Every call goes to libparhip.so.
     compute_recovery_full(...same...)                               gf64_addon.cc:1443
     compute_recovery_accumulate(...same...)  (accumulate=true)      gf64_addon.cc:1799
     compute_recovery_barycentric(...same...[, accumulate])          gf64_addon.cc:1636
+    compute_recovery_fenger(...same...)                             gf64_addon.cc:1980
     compute_recovery_streaming(inputBatches, outputBuffer, numInputs, numRecovery,
                      blockSize, firstInput, firstRecovery[, numThreads])
                                                                     gf64_addon.cc:2349
@@ -95,6 +96,21 @@ def compute_recovery_barycentric(inputs, outputs, numInputs, numRecovery, blockS
     check(lib().parhip_gf64_compute_recovery_blocks_barycentric(
         _buf(inputs, "inputs"), numInputs, _buf(outputs, "outputs"), numRecovery, blockSize // 8,
         firstInput, firstRecovery, numThreads, 1 if accumulate else 0))
+    return outputs
+
+
+def compute_recovery_fenger(inputs, outputs, numInputs, numRecovery, blockSize, firstInput,
+                            firstRecovery, numThreads=0):
+    """Negative counts are a RangeError, zero counts a no-op (gf64_addon.cc:2017-2055,
+    src/par3_engine_fenger.cc:118-119)."""
+    if numInputs < 0 or numRecovery < 0:
+        raise ValueError("numInputs and numRecovery must be non-negative")
+    _check_block(blockSize)
+    _check_sizes(inputs, numInputs, blockSize, "inputs")
+    _check_sizes(outputs, numRecovery, blockSize, "outputs")
+    check(lib().parhip_gf64_compute_recovery_blocks_fenger(
+        _buf(inputs, "inputs"), numInputs, _buf(outputs, "outputs"), numRecovery, blockSize // 8,
+        firstInput, firstRecovery, numThreads))
     return outputs
 
 

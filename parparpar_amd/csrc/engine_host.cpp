@@ -268,6 +268,14 @@ int parhip_gf64_compute_recovery_blocks_barycentric(const uint64_t* inputs, size
 	                                           accumulate);
 }
 
+int parhip_gf64_compute_recovery_blocks_fenger(const uint64_t* inputs, size_t numInputs,
+                                               uint64_t* recovery, size_t numRecovery,
+                                               size_t blockSize64, uint64_t firstInput,
+                                               uint64_t firstRecovery, int numThreads) {
+	return parhip_gf64_compute_recovery_blocks(inputs, numInputs, recovery, numRecovery,
+	                                           blockSize64, firstInput, firstRecovery, numThreads, 0);
+}
+
 int parhip_gf64_compute_recovery_batches(const uint64_t* const* batches, const size_t* batchBlocks,
                                          size_t numBatches, uint64_t* recovery, size_t numInputs,
                                          size_t numRecovery, size_t blockSize64,

@@ -5,7 +5,9 @@ GF64Controller::ComputeRecoveryBlocksBarycentric, whose two branches (R <= 32
 delegation, R > 32 per-row inverse + muladd_arr,
 src/par3_engine_barycentric.cc:121-229) both compute the Cauchy product;
 compute_recovery_streaming (src/gf64_addon.cc:2349-2650) concatenates a list of
-input batches and computes the same product once.  Cases: R on both sides of
+input batches and computes the same product once; compute_recovery_fenger
+(:1980-2078) is documented by the reference as bit-exact with
+ComputeRecoveryBlocks.  Cases: R on both sides of
 32, accumulate, uneven and empty batches, batches that straddle staging chunks,
 and the argument errors of both bindings.
 """
@@ -33,6 +35,23 @@ def test_barycentric(gpu, oracle, n, w, r, fi, fr):
     acc = prev.copy()
     binding.compute_recovery_barycentric(inp, acc, n, r, w * 8, fi, fr, 0, True)
     assert np.array_equal(acc, prev ^ want)
+
+
+def test_fenger(gpu, oracle):
+    """compute_recovery_fenger writes the Cauchy product (R = 9, 16, and 8192 where the
+    reference starts to choose it); zero counts leave the outputs alone."""
+    from parparpar_amd import binding
+    rng = np.random.default_rng(5)
+    for n, w, r in [(12, 64, 9), (33, 40, 16), (4, 16, 8192)]:
+        inp = rand_words(rng, n, w)
+        out = rand_words(rng, r, w)
+        binding.compute_recovery_fenger(inp, out, n, r, w * 8, 0, n)
+        assert np.array_equal(out, oracle.recovery(inp, r, 0, n))
+    keep = np.full((2, 8), 9, dtype=np.uint64)
+    binding.compute_recovery_fenger(keep, keep, 0, 2, 64, 0, 0)
+    assert (keep == 9).all()
+    with pytest.raises(ValueError):
+        binding.compute_recovery_fenger(keep, keep, -1, 2, 64, 0, 0)
 
 
 def test_barycentric_errors(gpu):
