@@ -43,8 +43,9 @@ parhip_batch *parhip_batch_create(const parhip_gf64_job *jobs, size_t njobs, uns
 int parhip_batch_run(parhip_batch *b, void *stream);
 int parhip_batch_info(const parhip_batch *b, uint64_t *ntiles, uint64_t *natomic_jobs,
                       uint64_t *coeff_bytes);
-/* Launch variant chosen for job `job`: outputs per work-group, words per
- * thread, inputs per LDS batch. */
+/* Launch variant chosen for job `job` (its index in the array given to
+ * parhip_batch_create): outputs per work-group, words per thread, inputs per
+ * LDS batch.  PARHIP_ERR_ARG for an empty job (no inputs, outputs or words). */
 int parhip_batch_job_config(const parhip_batch *b, size_t job, int *rt, int *wpt, int *ib);
 void parhip_batch_destroy(parhip_batch *b);
 

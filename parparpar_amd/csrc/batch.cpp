@@ -200,8 +200,9 @@ using namespace parhip;
 // ====================================================================== batch object
 struct parhip_batch {
 	int dev = 0;
-	std::vector<BatchJob> jobs;
+	std::vector<BatchJob> jobs;  // the non-empty jobs, in order
 	std::vector<int> cfg;
+	std::vector<int64_t> slot;   // caller's job index -> index in jobs, -1 if empty
 	BatchJob* djobs = nullptr;
 	BatchTile* dtiles = nullptr;
 	uint32_t ntiles = 0;
@@ -252,6 +253,7 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 		return nullptr;
 	}
 	// ---- job descriptors + coefficient storage
+	b->slot.assign(njobs, -1);
 	for (size_t j = 0; j < njobs; j++) {
 		const parhip_gf64_job& in = jobs[j];
 		if (in.numInputs == 0 || in.numRecovery == 0 || in.blockSize64 == 0) continue;
@@ -273,6 +275,7 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 		jb.fr = in.firstRecovery;
 		jb.coeff_off = b->coeff_total;
 		b->coeff_total += jb.N * jb.R;
+		b->slot[j] = (int64_t)b->jobs.size();
 		b->jobs.push_back(jb);
 	}
 	if (b->jobs.empty()) return b;  // nothing to do: run() is a no-op
@@ -285,11 +288,17 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 		return nullptr;
 	}
 	for (auto& jb : b->jobs) jb.coeff = b->dcoeff + jb.coeff_off;
-	hipMemcpy(b->djobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice);
+	e = hipMemcpy(b->djobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice);
 	// coefficients first: the autotuner runs the real kernel on them
-	e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), nullptr, 0, b->dcounter, nullptr, 0,
-	                 b->coeff_total, 0, b->cap);
-	hipStreamSynchronize(b->cap);
+	if (e == hipSuccess)
+		e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), nullptr, 0, b->dcounter, nullptr, 0,
+		                 b->coeff_total, 0, b->cap);
+	if (e == hipSuccess) e = hipStreamSynchronize(b->cap);
+	if (e != hipSuccess) {
+		hip_fail(e, "batch coefficients");
+		batch_free(b);
+		return nullptr;
+	}
 	// ---- per-job launch variant
 	b->cfg.resize(b->jobs.size());
 	for (size_t j = 0; j < b->jobs.size(); j++) {
@@ -334,10 +343,15 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 		batch_free(b);
 		return nullptr;
 	}
-	hipMemcpy(b->dtiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice);
-	if (b->nzero)
-		hipMemcpy(b->dzero, zero_ids.data(), zero_ids.size() * sizeof(uint32_t),
-		          hipMemcpyHostToDevice);
+	e = hipMemcpy(b->dtiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice);
+	if (e == hipSuccess && b->nzero)
+		e = hipMemcpy(b->dzero, zero_ids.data(), zero_ids.size() * sizeof(uint32_t),
+		              hipMemcpyHostToDevice);
+	if (e != hipSuccess) {
+		hip_fail(e, "batch tile upload");
+		batch_free(b);
+		return nullptr;
+	}
 	// ---- graph: Cauchy build, zeroing, queue reset, persistent kernel
 	if (flags & PARHIP_BATCH_GRAPH) {
 		e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
@@ -381,8 +395,9 @@ int parhip_batch_info(const parhip_batch* b, uint64_t* ntiles, uint64_t* natomic
 }
 
 int parhip_batch_job_config(const parhip_batch* b, size_t job, int* rt, int* wpt, int* ib) {
-	if (!b || job >= b->cfg.size()) return fail(PARHIP_ERR_ARG, "batch_job_config: bad job");
-	const CfgInfo& c = kCfg[b->cfg[job]];
+	if (!b || job >= b->slot.size()) return fail(PARHIP_ERR_ARG, "batch_job_config: bad job");
+	if (b->slot[job] < 0) return fail(PARHIP_ERR_ARG, "batch_job_config: job %zu is empty", job);
+	const CfgInfo& c = kCfg[b->cfg[(size_t)b->slot[job]]];
 	if (rt) *rt = c.rt;
 	if (wpt) *wpt = c.wpt;
 	if (ib) *ib = c.ib;

@@ -287,6 +287,11 @@ Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev) {
 			p.ib = ib;
 		}
 	}
+	if (N == 0 || R == 0 || W == 0) {  // nothing to launch
+		p.groups = 1;
+		p.in_per_group = N;
+		return p;
+	}
 	const uint64_t tile = 256ull * p.wpt;
 	const uint64_t tiles = (W + tile - 1) / tile;
 	const uint64_t rtiles = (R + p.rt - 1) / p.rt;

@@ -30,17 +30,25 @@ int gf16_device_tables(const uint16_t** exp_tab, const uint16_t** inlog) {
 	std::lock_guard<std::mutex> lk(g_t16_mu);
 	int dev = current_device();
 	auto& t = g_t16[dev];
-	if (!t.exp_tab) {
+	if (!t.exp_tab) {  // published only once all three are uploaded
 		const Gf16HostTables& h = gf16_host_tables();
-		hipError_t e = hipMalloc((void**)&t.exp_tab, 65536 * sizeof(uint16_t));
-		if (e != hipSuccess) return hip_fail(e, "hipMalloc(gf16 tables)");
-		e = hipMalloc((void**)&t.inlog, 32768 * sizeof(uint16_t));
-		if (e != hipSuccess) return hip_fail(e, "hipMalloc(gf16 tables)");
-		e = hipMalloc((void**)&t.log_tab, 65536 * sizeof(uint16_t));
-		if (e != hipSuccess) return hip_fail(e, "hipMalloc(gf16 tables)");
-		hipMemcpy(t.exp_tab, h.exp, 65536 * sizeof(uint16_t), hipMemcpyHostToDevice);
-		hipMemcpy(t.inlog, h.inlog, 32768 * sizeof(uint16_t), hipMemcpyHostToDevice);
-		hipMemcpy(t.log_tab, h.log, 65536 * sizeof(uint16_t), hipMemcpyHostToDevice);
+		Gf16DevTables n;
+		hipError_t e = hipMalloc((void**)&n.exp_tab, 65536 * sizeof(uint16_t));
+		if (e == hipSuccess) e = hipMalloc((void**)&n.inlog, 32768 * sizeof(uint16_t));
+		if (e == hipSuccess) e = hipMalloc((void**)&n.log_tab, 65536 * sizeof(uint16_t));
+		if (e == hipSuccess)
+			e = hipMemcpy(n.exp_tab, h.exp, 65536 * sizeof(uint16_t), hipMemcpyHostToDevice);
+		if (e == hipSuccess)
+			e = hipMemcpy(n.inlog, h.inlog, 32768 * sizeof(uint16_t), hipMemcpyHostToDevice);
+		if (e == hipSuccess)
+			e = hipMemcpy(n.log_tab, h.log, 65536 * sizeof(uint16_t), hipMemcpyHostToDevice);
+		if (e != hipSuccess) {
+			hipFree(n.exp_tab);
+			hipFree(n.inlog);
+			hipFree(n.log_tab);
+			return hip_fail(e, "gf16 device tables");
+		}
+		t = n;
 	}
 	*exp_tab = t.exp_tab;
 	*inlog = t.inlog;

@@ -131,3 +131,15 @@ def test_streaming_errors(gpu):
     cnt = (ctypes.c_size_t * 1)(2)
     assert P.lib().parhip_gf64_compute_recovery_batches(ptrs, cnt, 1, out.ctypes.data, 3, 2, 8, 0, 3,
                                                         0) == P.PARHIP_ERR_RANGE
+
+
+def test_workspace_query_empty_shapes(gpu):
+    """The workspace query is defined for empty shapes (no division by an empty tile count)."""
+    import parparpar_amd as P
+    L = P.lib()
+    assert L.parhip_gf64_workspace_bytes(0, 8, 128) == 0
+    assert L.parhip_gf64_workspace_bytes(8, 0, 128) == 0
+    assert L.parhip_gf64_workspace_bytes(8, 8, 0) == 0
+    assert L.parhip_gf64_workspace_bytes(1024, 8, 131072) >= 0
+    assert L.parhip_gf64_repair_workspace_bytes(64, 512, 0) >= 0
+    assert L.parhip_gf64_repair_workspace_bytes(8, 512, 8) > 0

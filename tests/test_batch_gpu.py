@@ -70,3 +70,28 @@ def test_graph_replays_match_eager(gpu):
         for k, (_, y, _, _) in enumerate(jobs):
             assert torch.equal(y, refs[k]), "replay %d job %d" % (rep, k)
     b.close()
+
+
+def test_batch_skips_empty_jobs(gpu):
+    """Jobs with no inputs or no outputs are skipped; job indices still refer to
+    the caller's array."""
+    import torch
+    import parparpar_amd as P
+    from parparpar_amd import device
+    from parparpar_amd.batch import Batch
+    rng = np.random.default_rng(5)
+    shapes = [(3, 5, 2, 0, 3), (0, 64, 4, 0, 0), (10, 64, 0, 0, 10), (8, 256, 3, 0, 8)]
+    jobs = _jobs(rng, shapes, torch)
+    b = Batch(jobs, graph=False, autotune=False)
+    b.run()
+    torch.cuda.synchronize()
+    for k in (0, 3):
+        x, y, fi, fr = jobs[k]
+        assert torch.equal(y, device.recovery(x, y.shape[0], fi, fr)), k
+        assert b.job_config(k)[0] > 0
+    for k in (1, 2):
+        with pytest.raises(P.ParhipError):
+            b.job_config(k)
+    with pytest.raises(P.ParhipError):
+        b.job_config(4)
+    b.close()
