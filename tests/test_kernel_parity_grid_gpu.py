@@ -1,0 +1,211 @@
+"""The reference's randomized kernel parity grid, replayed against the oracle.
+
+test/par3-kernel-parity.js draws its cases from mulberry32 (seeds 0xDEADBEEF,
+0xFEEDFACE, 0xBADDCAFE) and checks compute_recovery and Gf64Encoder.mul_arr
+against a BigInt restatement.  The same draws are made here with a restatement
+of mulberry32 (the published 32-bit generator: add 0x6D2B79F5, two
+xor-shift-multiply rounds), so the case list follows the reference's; the
+expected bytes come from the oracle, so the checks do not depend on the
+generator matching bit for bit.
+
+  A  1000 cases: numInputs {4, 16, 64}, numRecovery {1, 4, 16}, blockSize
+     {8, 64, 1024} bytes, firstInput [0, 100], firstRecovery [200, 300]
+  B  the first 50 of them at numThreads {0, 1, 2, 4}: identical bytes
+  C  mul_arr with n_coeff {2, 4, 8, 16, 32} x blockSize {64, 256, 1024}: cycling
+     coefficients out[w] = in[w] * coeff[w % n_coeff]
+  D  mul_arr with n_coeff {1, 4, 64} x {64, 1024, 4096} bytes x 30 trials
+"""
+import numpy as np
+import pytest
+
+M32 = 0xFFFFFFFF
+
+
+def mulberry32(seed):
+    state = [seed & M32]
+
+    def nxt():  # next uint32 (the reference's rng() * 2^32)
+        s = (state[0] + 0x6D2B79F5) & M32
+        state[0] = s
+        t = ((s ^ (s >> 15)) * (1 | s)) & M32
+        t = ((t + (((t ^ (t >> 7)) * (61 | t)) & M32)) & M32) ^ t
+        return (t ^ (t >> 14)) & M32
+
+    return nxt
+
+
+def pick(nxt, pool):
+    return pool[(nxt() * len(pool)) >> 32]
+
+
+def words(nxt, n):
+    out = np.empty(n, dtype=np.uint64)
+    for w in range(n):
+        hi = nxt()
+        lo = nxt()
+        out[w] = (hi << 32) | lo
+    return out
+
+
+def section_a_cases(count):
+    nxt = mulberry32(0xDEADBEEF)
+    cases = []
+    for _ in range(count):
+        ni = pick(nxt, [4, 16, 64])
+        nr = pick(nxt, [1, 4, 16])
+        bs = pick(nxt, [8, 64, 1024])
+        fi = (nxt() * 101) >> 32
+        fr = ((nxt() * 101) >> 32) + 200
+        inp = words(nxt, ni * bs // 8).reshape(ni, bs // 8)
+        cases.append((ni, nr, bs, fi, fr, inp))
+    return cases
+
+
+def test_mulberry32_draw_ranges():
+    """The restated generator yields uint32 values and the case fields stay in
+    the reference's pools (CPU only)."""
+    cases = section_a_cases(200)
+    assert {c[0] for c in cases} == {4, 16, 64}
+    assert {c[1] for c in cases} == {1, 4, 16}
+    assert {c[2] for c in cases} == {8, 64, 1024}
+    assert all(0 <= c[3] <= 100 and 200 <= c[4] <= 300 for c in cases)
+
+
+@pytest.mark.gpu
+def test_section_a_compute_recovery(gpu, oracle):
+    from parparpar_amd import binding
+    bad = []
+    for k, (ni, nr, bs, fi, fr, inp) in enumerate(section_a_cases(1000)):
+        out = np.zeros((nr, bs // 8), dtype=np.uint64)
+        binding.compute_recovery(inp, out, ni, nr, bs, fi, fr, 0)
+        if not np.array_equal(out, oracle.recovery(inp, nr, fi, fr)):
+            bad.append((k + 1, ni, nr, bs, fi, fr))
+    assert not bad, bad[:5]
+
+
+@pytest.mark.gpu
+def test_section_b_thread_counts(gpu):
+    from parparpar_amd import binding
+    for (ni, nr, bs, fi, fr, inp) in section_a_cases(50):
+        ref = np.zeros((nr, bs // 8), dtype=np.uint64)
+        binding.compute_recovery(inp, ref, ni, nr, bs, fi, fr, 0)
+        for nt in (0, 1, 2, 4):
+            out = np.zeros_like(ref)
+            binding.compute_recovery(inp, out, ni, nr, bs, fi, fr, nt)
+            assert np.array_equal(out, ref), (ni, nr, bs, nt)
+
+
+def _mul_arr_expected(oracle, inp, coeff):
+    exp = np.zeros_like(inp)
+    oracle.lib().or_gf64_region_mul_arr(oracle.p64(exp), oracle.p64(inp), oracle.p64(coeff), inp.size,
+                                        coeff.size)
+    return exp
+
+
+@pytest.mark.gpu
+def test_section_c_mul_arr_cycling(gpu, oracle):
+    from parparpar_amd import binding
+    nxt = mulberry32(0xFEEDFACE)
+    for n_coeff in (2, 4, 8, 16, 32):
+        for bs in (64, 256, 1024):
+            nw = bs // 8
+            inp = words(nxt, nw)
+            coeff = words(nxt, n_coeff)
+            out = np.zeros(nw, dtype=np.uint64)
+            binding.Gf64Encoder().mul_arr(out, inp, coeff, nw, n_coeff)
+            assert np.array_equal(out, _mul_arr_expected(oracle, inp, coeff)), (n_coeff, bs)
+
+
+@pytest.mark.gpu
+def test_section_d_mul_arr_trials(gpu, oracle):
+    from parparpar_amd import binding
+    nxt = mulberry32(0xBADDCAFE)
+    enc = binding.Gf64Encoder()
+    for n_coeff in (1, 4, 64):
+        for nbytes in (64, 1024, 4096):
+            nw = nbytes // 8
+            for trial in range(30):
+                inp = words(nxt, nw)
+                coeff = words(nxt, n_coeff)
+                out = np.zeros(nw, dtype=np.uint64)
+                enc.mul_arr(out, inp, coeff, nw, n_coeff)
+                assert np.array_equal(out, _mul_arr_expected(oracle, inp, coeff)), (n_coeff, nbytes, trial)
+
+
+# test/par3-small-r-fused-parity.js:77-82 (compute_recovery_full at several
+# thread counts, then compute_recovery_barycentric on fresh inputs)
+SMALL_R = [(32, 1, 1024), (32, 2, 1024), (64, 4, 2048), (128, 8, 4096), (128, 16, 2048), (64, 32, 1024)]
+
+
+@pytest.mark.gpu
+def test_small_r_fused_configs(gpu, oracle):
+    from parparpar_amd import binding
+    nxt = mulberry32(0x5A110F05ED)
+    for (n, r, b) in SMALL_R:
+        inp = words(nxt, n * b // 8).reshape(n, b // 8)
+        ref = np.zeros((r, b // 8), dtype=np.uint64)
+        binding.compute_recovery_full(inp, ref, n, r, b, 0, n, 1)
+        assert np.array_equal(ref, oracle.recovery(inp, r, 0, n)), (n, r, b)
+        for threads in (2, 4, 8, 16):
+            out = np.zeros_like(ref)
+            binding.compute_recovery_full(inp, out, n, r, b, 0, n, threads)
+            assert np.array_equal(out, ref), (n, r, b, threads)
+    for (n, r, b) in SMALL_R:
+        inp = words(nxt, n * b // 8).reshape(n, b // 8)
+        full = np.zeros((r, b // 8), dtype=np.uint64)
+        bary = np.zeros_like(full)
+        binding.compute_recovery_full(inp, full, n, r, b, 0, n, 8)
+        binding.compute_recovery_barycentric(inp, bary, n, r, b, 0, n, 8)
+        assert np.array_equal(full, bary) and np.array_equal(bary, oracle.recovery(inp, r, 0, n))
+
+
+def fill_pattern(nbytes):
+    """test/par3-2d-decomposition-parity.js fillPattern: the uint32 at byte i is
+    (i * 1103515245 + 12345) >>> 0, evaluated in IEEE doubles as JavaScript does."""
+    i = np.arange(0, nbytes, 4, dtype=np.float64)
+    v = (i * 1103515245.0 + 12345.0).astype(np.uint64) & np.uint64(M32)
+    return v.astype(np.uint32).view(np.uint64)
+
+
+# test/par3-2d-decomposition-parity.js:42-54: the reference's S_R x S_B
+# decomposition at many thread counts, incl. R > 32 (K x G tiles)
+DECOMP = [(8, 2, 128 << 10, (1, 2, 4, 8, 16, 32)), (16, 4, 256 << 10, (1, 2, 4, 8, 16, 32, 64)),
+          (32, 8, 512 << 10, (1, 2, 8, 16, 32, 64)), (16, 2, 256 << 10, (3, 5, 6, 7, 12)),
+          (4, 2, 1 << 20, (1, 4, 8, 16)), (16, 64, 256 << 10, (1, 2, 32, 64, 128))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,r,b,threads", DECOMP)
+def test_2d_decomposition_shapes(gpu, oracle, n, r, b, threads):
+    from parparpar_amd import binding
+    inp = fill_pattern(n * b).reshape(n, b // 8)
+    want = oracle.recovery(inp, r, 0, n)
+    for t in threads:
+        out = np.zeros((r, b // 8), dtype=np.uint64)
+        binding.compute_recovery_full(inp, out, n, r, b, 0, n, t)
+        assert np.array_equal(out, want), t
+
+
+@pytest.mark.gpu
+def test_barycentric_grid_and_bit_flip(gpu, oracle):
+    """test/par3-barycentric-parity.js:216-231 grid (N {16, 64, 256} x R {1, 4, 8}
+    x blockSize {8, 64, 1024}) and its negative trap (:426-470): one flipped
+    input bit changes the output, which still equals the oracle."""
+    from parparpar_amd import binding
+    nxt = mulberry32(0xBA7C)
+    for n in (16, 64, 256):
+        for r in (1, 4, 8):
+            for b in (8, 64, 1024):
+                inp = words(nxt, n * b // 8).reshape(n, b // 8)
+                out = np.zeros((r, b // 8), dtype=np.uint64)
+                binding.compute_recovery_barycentric(inp, out, n, r, b, 0, n + 1)
+                assert np.array_equal(out, oracle.recovery(inp, r, 0, n + 1)), (n, r, b)
+    n, r, b = 64, 4, 64
+    inp = words(nxt, n * b // 8).reshape(n, b // 8)
+    base = np.zeros((r, b // 8), dtype=np.uint64)
+    binding.compute_recovery_barycentric(inp, base, n, r, b, 0, n + 1)
+    inp[0, 0] ^= np.uint64(1 << 17)
+    flipped = np.zeros_like(base)
+    binding.compute_recovery_barycentric(inp, flipped, n, r, b, 0, n + 1)
+    assert not np.array_equal(flipped, base)
+    assert np.array_equal(flipped, oracle.recovery(inp, r, 0, n + 1))
