@@ -14,6 +14,10 @@ generator matching bit for bit.
   C  mul_arr with n_coeff {2, 4, 8, 16, 32} x blockSize {64, 256, 1024}: cycling
      coefficients out[w] = in[w] * coeff[w % n_coeff]
   D  mul_arr with n_coeff {1, 4, 64} x {64, 1024, 4096} bytes x 30 trials
+
+and, in the same way, the grids of test/par3-small-r-fused-parity.js,
+par3-2d-decomposition-parity.js, par3-barycentric-parity.js,
+par3-accumulate-binding-parity.js and par3-full-recovery-parity.js.
 """
 import numpy as np
 import pytest
@@ -209,3 +213,81 @@ def test_barycentric_grid_and_bit_flip(gpu, oracle):
     binding.compute_recovery_barycentric(inp, flipped, n, r, b, 0, n + 1)
     assert not np.array_equal(flipped, base)
     assert np.array_equal(flipped, oracle.recovery(inp, r, 0, n + 1))
+
+
+# test/par3-accumulate-binding-parity.js:52-60: chunked accumulation over
+# dirty outputs (first chunk writes, later chunks XOR), for the full and
+# barycentric entries, across the R = 32 threshold
+ACCUM = [(16, 2, 64, 2), (32, 4, 512, 4), (64, 8, 4096, 4), (128, 8, 16384, 8), (256, 16, 4096, 4),
+         (64, 32, 1024, 2), (48, 34, 512, 3)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("entry", ["full", "barycentric"])
+def test_chunked_accumulation(gpu, oracle, entry):
+    from parparpar_amd import binding
+    fn = {"full": binding.compute_recovery_full, "barycentric": binding.compute_recovery_barycentric}[entry]
+    nxt = mulberry32(0x110BADC0DE if entry == "full" else 0x110BADC0DF)
+    for (ni, nr, bs, chunks) in ACCUM:
+        inp = words(nxt, ni * bs // 8).reshape(ni, bs // 8)
+        single = np.zeros((nr, bs // 8), dtype=np.uint64)
+        fn(inp, single, ni, nr, bs, 0, ni, 0, False)
+        assert np.array_equal(single, oracle.recovery(inp, nr, 0, ni)), (ni, nr, bs)
+        out = np.full((nr, bs // 8), 0xAAAAAAAAAAAAAAAA, dtype=np.uint64)
+        per = ni // chunks
+        for c in range(chunks):
+            s = c * per
+            e = ni if c == chunks - 1 else (c + 1) * per
+            sub = np.ascontiguousarray(inp[s:e])
+            fn(sub, out, e - s, nr, bs, s, ni, 0, c > 0)
+        assert np.array_equal(out, single), (entry, ni, nr, bs, chunks)
+
+
+@pytest.mark.gpu
+def test_accumulate_alias_and_trap(gpu):
+    """:139-176: compute_recovery_accumulate on zeroed outputs equals
+    compute_recovery_full; a flipped input bit is seen by the chunked path, and
+    a chunk that overwrites instead of accumulating is caught."""
+    from parparpar_amd import binding
+    nxt = mulberry32(0x7AA9)
+    t = words(nxt, 8 * 8).reshape(8, 8)
+    a = np.zeros((2, 8), dtype=np.uint64)
+    b = np.zeros((2, 8), dtype=np.uint64)
+    binding.compute_recovery_full(t, a, 8, 2, 64, 0, 8, 0, False)
+    binding.compute_recovery_accumulate(t, b, 8, 2, 64, 0, 8, 0)
+    assert np.array_equal(a, b)
+    clean = words(nxt, 16 * 8).reshape(16, 8)
+    ref = np.zeros((4, 8), dtype=np.uint64)
+    binding.compute_recovery_full(clean, ref, 16, 4, 64, 0, 16, 0, False)
+    bad = clean.copy()
+    bad.view(np.uint8).reshape(-1)[9 * 64 + 3] ^= 1
+    single = np.zeros_like(ref)
+    binding.compute_recovery_full(bad, single, 16, 4, 64, 0, 16, 0, False)
+    chunked = np.zeros_like(ref)
+    binding.compute_recovery_full(np.ascontiguousarray(bad[:8]), chunked, 8, 4, 64, 0, 16, 0, False)
+    binding.compute_recovery_full(np.ascontiguousarray(bad[8:]), chunked, 8, 4, 64, 8, 16, 0, True)
+    flawed = np.zeros_like(ref)
+    binding.compute_recovery_full(np.ascontiguousarray(bad[:8]), flawed, 8, 4, 64, 0, 16, 0, False)
+    binding.compute_recovery_full(np.ascontiguousarray(bad[8:]), flawed, 8, 4, 64, 8, 16, 0, False)
+    assert not np.array_equal(ref, single)
+    assert np.array_equal(chunked, single)
+    assert not np.array_equal(chunked, flawed)
+
+
+@pytest.mark.gpu
+def test_full_recovery_random_configs(gpu, oracle):
+    """test/par3-full-recovery-parity.js:127-160: 16 draws of numInputs {4, 16, 64},
+    numRecovery {2, 8, 16}, blockSize {4096, 16384}, firstInput [0, 99],
+    firstRecovery [200, 299]."""
+    from parparpar_amd import binding
+    nxt = mulberry32(0xC0FFEE42)
+    for _ in range(16):
+        ni = pick(nxt, [4, 16, 64])
+        nr = pick(nxt, [2, 8, 16])
+        bs = pick(nxt, [4096, 16384])
+        fi = (nxt() * 100) >> 32
+        fr = ((nxt() * 100) >> 32) + 200
+        inp = words(nxt, ni * bs // 8).reshape(ni, bs // 8)
+        out = np.zeros((nr, bs // 8), dtype=np.uint64)
+        binding.compute_recovery_full(inp, out, ni, nr, bs, fi, fr, 0)
+        assert np.array_equal(out, oracle.recovery(inp, nr, fi, fr)), (ni, nr, bs, fi, fr)
