@@ -25,8 +25,10 @@ reference's own call sequences.  Every call goes to libparhip.so.
     Gf64Encoder().{mul, mul_arr, coupled_muladd_arr, fused_output_muladd_arr,
                    two_d_muladd_arr}                                gf64_addon.cc:168-1054
 
-`blockSize` is in bytes and must be a multiple of 8, as the addon requires
-(gf64_addon.cc:1553-1578); violations raise ValueError (the addon's RangeError).
+`blockSize` is in bytes and must be a multiple of 8, and the counts must be
+positive, as the addon requires (gf64_addon.cc:1553-1578); violations raise
+ValueError (the addon's RangeError).  The C entries underneath treat empty
+problems as no-ops, as GF64Controller does (src/par3_engine.cc:1069).
 """
 import ctypes
 
@@ -53,8 +55,18 @@ def _check_block(blockSize):
         raise ValueError("blockSize must be a positive multiple of 8")
 
 
+def _check_counts(numInputs, numRecovery):
+    # the addon rejects empty problems (gf64_addon.cc:1383-1391, :1554-1562,
+    # :1917-1922, :2153-2156); the engine underneath treats them as no-ops
+    if numInputs <= 0:
+        raise ValueError("numInputs must be positive")
+    if numRecovery <= 0:
+        raise ValueError("numRecovery must be positive")
+
+
 def compute_recovery(inputs, outputs, numInputs, numRecovery, blockSize, firstInput, firstRecovery,
                      numThreads=0, accumulate=False):
+    _check_counts(numInputs, numRecovery)
     _check_block(blockSize)
     _check_sizes(inputs, numInputs, blockSize, "inputs")
     _check_sizes(outputs, numRecovery, blockSize, "outputs")
@@ -68,6 +80,7 @@ def compute_recovery(inputs, outputs, numInputs, numRecovery, blockSize, firstIn
 
 def compute_recovery_full(inputs, outputs, numInputs, numRecovery, blockSize, firstInput,
                           firstRecovery, numThreads=0, accumulate=False):
+    _check_counts(numInputs, numRecovery)
     _check_block(blockSize)
     _check_sizes(inputs, numInputs, blockSize, "inputs")
     _check_sizes(outputs, numRecovery, blockSize, "outputs")
@@ -86,10 +99,7 @@ def compute_recovery_accumulate(inputs, outputs, numInputs, numRecovery, blockSi
 def compute_recovery_barycentric(inputs, outputs, numInputs, numRecovery, blockSize, firstInput,
                                  firstRecovery, numThreads=0, accumulate=False):
     """Counts must be positive, as the addon requires (gf64_addon.cc:1746-1755)."""
-    if numInputs <= 0:
-        raise ValueError("numInputs must be positive")
-    if numRecovery <= 0:
-        raise ValueError("numRecovery must be positive")
+    _check_counts(numInputs, numRecovery)
     _check_block(blockSize)
     _check_sizes(inputs, numInputs, blockSize, "inputs")
     _check_sizes(outputs, numRecovery, blockSize, "outputs")
@@ -155,6 +165,7 @@ def compute_recovery_streaming(inputBatches, outputBuffer, numInputs, numRecover
 
 def compute_recovery_with_coeff(coeffMatrix, inputs, outputs, numInputs, numRecovery, blockSize,
                                 firstInput=0, firstRecovery=0, numThreads=0, accumulate=False):
+    _check_counts(numInputs, numRecovery)
     _check_block(blockSize)
     _check_sizes(inputs, numInputs, blockSize, "inputs")
     _check_sizes(outputs, numRecovery, blockSize, "outputs")
@@ -166,6 +177,8 @@ def compute_recovery_with_coeff(coeffMatrix, inputs, outputs, numInputs, numReco
 
 
 def build_coefficient_matrix(numInputs, numRecovery, firstInput, firstRecovery):
+    if numInputs <= 0 or numRecovery <= 0:
+        raise ValueError("numInputs and numRecovery must be positive")
     out = np.empty(numInputs * numRecovery, dtype=np.uint64)
     check(lib().parhip_gf64_build_cauchy_matrix(out.ctypes.data, numInputs, numRecovery,
                                                 firstInput, firstRecovery))

@@ -226,10 +226,51 @@ def test_input_errors(gpu):
     from parparpar_amd import binding
     with pytest.raises(ValueError):
         binding.compute_recovery(np.zeros(2, np.uint64), np.zeros(2, np.uint64), 2, 1, 7, 0, 2)
-    # N == 0 / R == 0: silent no-op like the reference (src/par3_engine.cc:1069)
+    # N == 0 / R == 0: the addon throws RangeError and leaves the output alone
+    # (test/par3-native-edge.js scenarios 1-2); the engine entry is a silent
+    # no-op (src/par3_engine.cc:1069)
+    import parparpar_amd as P
     out = np.full(4, 5, dtype=np.uint64)
-    binding.compute_recovery(np.zeros(4, np.uint64), out, 0, 1, 32, 0, 0)
+    for n, r in [(0, 1), (1, 0), (-1, 1)]:
+        with pytest.raises(ValueError):
+            binding.compute_recovery(np.zeros(4, np.uint64), out, n, r, 32, 0, 0)
+        with pytest.raises(ValueError):
+            binding.compute_recovery_full(np.zeros(4, np.uint64), out, n, r, 32, 0, 0)
+    with pytest.raises(ValueError):
+        binding.build_coefficient_matrix(0, 2, 0, 0)
     assert (out == 5).all()
+    assert P.lib().parhip_gf64_compute_recovery_blocks(out.ctypes.data, 0, out.ctypes.data, 1, 4, 0, 0, 0,
+                                                       0) == 0
+    assert (out == 5).all()
+    # scenarios 5-8, 13: bad block sizes, an impossible numInputs, a short output
+    with pytest.raises(ValueError):
+        binding.compute_recovery(np.zeros(2, np.uint64), np.zeros(1, np.uint64), 2, 1, 0, 0, 2)
+    with pytest.raises(ValueError):
+        binding.compute_recovery(np.zeros(128, np.uint64), np.zeros(128, np.uint64), 2147483647, 1, 8, 0, 1)
+    with pytest.raises(ValueError):
+        binding.compute_recovery(np.zeros(2, np.uint64), np.zeros(4, np.uint8), 2, 1, 8, 0, 2)
+
+
+def test_output_larger_than_needed_and_state_isolation(gpu, oracle):
+    """test/par3-native-edge.js scenarios 14 and 18: bytes past numRecovery x
+    blockSize are not written; a small call between two large ones changes
+    nothing in the second."""
+    from parparpar_amd import binding
+    rng = np.random.default_rng(18)
+    inp = rand_words(rng, 2, 1)
+    out = np.full(2, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
+    binding.compute_recovery(inp, out, 2, 1, 8, 0, 2, 1)
+    assert out[1] == 0xFFFFFFFFFFFFFFFF and out[0] == oracle.recovery(inp, 1, 0, 2)[0, 0]
+    big = rand_words(rng, 32, 131072)
+    o1 = np.zeros((10, 131072), dtype=np.uint64)
+    binding.compute_recovery(big, o1, 32, 10, 1 << 20, 0, 32)
+    small = np.zeros((1, 1), dtype=np.uint64)
+    binding.compute_recovery(rand_words(rng, 2, 1), small, 2, 1, 8, 0, 2, 1)
+    o2 = np.zeros_like(o1)
+    binding.compute_recovery(big, o2, 32, 10, 1 << 20, 0, 32)
+    assert np.array_equal(o1, o2)
+    cols = [0, 1, 65535, 131071]
+    assert np.array_equal(o1[:, cols], oracle.recovery_columns(big, oracle.cauchy(32, 10, 0, 32), cols))
 
 
 # ----------------------------------------------------------- device-resident, full size

@@ -291,3 +291,31 @@ def test_full_recovery_random_configs(gpu, oracle):
         out = np.zeros((nr, bs // 8), dtype=np.uint64)
         binding.compute_recovery_full(inp, out, ni, nr, bs, fi, fr, 0)
         assert np.array_equal(out, oracle.recovery(inp, nr, fi, fr)), (ni, nr, bs, fi, fr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,n,bs", [(12, 3, 256), (40, 8, 1024), (64, 16, 512)])
+def test_repair_native_matrix_flow(gpu, oracle, N, n, bs):
+    """The repair flow of test/par3-repair-native-matrix.js:100-160 (and
+    lib/par3gen.js:3840-4012) through the host entries: one
+    build_coefficient_matrix (n rows x N columns), the A submatrix at the
+    missing columns, RHS = recovery ^ sum over known inputs of c * D (here one
+    compute_recovery_with_coeff over the known columns), solve_and_reconstruct;
+    the solution is the lost inputs."""
+    from parparpar_amd import binding
+    rng = np.random.default_rng(N * n)
+    fi, fr = 0, N
+    data = rng.integers(0, 2 ** 64, size=(N, bs // 8), dtype=np.uint64)
+    rec = np.zeros((n, bs // 8), dtype=np.uint64)
+    binding.compute_recovery(data, rec, N, n, bs, fi, fr)
+    missing = np.sort(rng.choice(N, n, replace=False))
+    known = np.setdiff1d(np.arange(N), missing)
+    M = binding.build_coefficient_matrix(N, n, fi, fr).reshape(n, N)
+    assert np.array_equal(M, oracle.cauchy(N, n, fi, fr))
+    A = np.ascontiguousarray(M[:, missing])
+    ck = np.ascontiguousarray(M[:, known])
+    rhs = rec.copy()
+    binding.compute_recovery_with_coeff(ck, np.ascontiguousarray(data[known]), rhs, len(known), n, bs,
+                                        accumulate=True)
+    assert binding.solve_and_reconstruct(A, rhs, n, bs) is True
+    assert np.array_equal(rhs, data[missing])
