@@ -87,38 +87,44 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 //               deterministic).
 enum { OUT_DIRECT = 0, OUT_PARTIAL = 1, OUT_ATOMIC = 2 };
 
+// Loads of one batch of IB inputs for mac_tab_tile: this thread's words of
+// each input (zeros past the input range or the slice end) and the two
+// coefficients of the table entry it builds (thread t < IB*NP*16 owns entry
+// (ib, p, n) = (t/16/NP, t/16%NP, t%16)).  The coefficient loads are issued
+// first, so a wait for them leaves the data loads in flight.  `fast`: strided,
+// 16-byte aligned slices and a tile that lies inside the slice (no per-lane checks).
 template <int RT, int WPT, int IB>
-__device__ __forceinline__ void mac_tab_tile(const MacArgs& a, uint4* tab, uint64_t w0,
-                                             uint64_t r0, uint64_t i_begin, uint64_t i_end,
-                                             uint64_t g, int out_mode) {
-	static_assert(RT % 2 == 0 && RT >= 2 && RT <= 16, "RT");
-	static_assert(WPT % 2 == 0 && WPT >= 2 && WPT <= 8, "WPT");
-	constexpr int NP = RT / 2;  // output pairs
-	// tab[((ib*NP + p)*16 + k)*16 + n] = (c_{2p} * n x^{4k}, c_{2p+1} * n x^{4k})
-#define TAB(ib, p, k, n) tab[(((ib) * NP + (p)) * 16 + (k)) * 16 + (n)]
+__device__ __forceinline__ void mac_tab_load(const MacArgs& a, uint64_t ib0, uint64_t i_end,
+                                             uint64_t w0, uint64_t r0, bool fast,
+                                             uint2 (&d)[IB][WPT], uint64_t& c0, uint64_t& c1) {
+	constexpr int NP = RT / 2;
 	const int tid = threadIdx.x;
-
-	// word positions owned by this thread: pairs (2*tid + 512*j, +1);
-	// acc[j][p] = (lo, hi) of output 2p and (lo, hi) of output 2p+1
-	uint4 acc[WPT][NP];
+	c0 = 0;
+	c1 = 0;
+	if (tid < IB * NP * 16) {
+		const int p = (tid >> 4) % NP;
+		const int ib = (tid >> 4) / NP;
+		const uint64_t i = ib0 + ib;
+		if (i < i_end) {
+			const uint64_t ra = r0 + 2 * p, rb = ra + 1;
+			if (ra < a.R) c0 = gload(a.coeff + ra * a.coeff_ld + i);
+			if (rb < a.R) c1 = gload(a.coeff + rb * a.coeff_ld + i);
+		}
+	}
 #pragma unroll
-	for (int j = 0; j < WPT; j++)
+	for (int ib = 0; ib < IB; ib++) {
+		const uint64_t i = ib0 + ib;
 #pragma unroll
-		for (int p = 0; p < NP; p++) acc[j][p] = make_uint4(0, 0, 0, 0);
-
-	for (uint64_t ib0 = i_begin; ib0 < i_end; ib0 += IB) {
-		// ---- issue data loads for this batch (in flight during the table build)
-		uint2 d[IB][WPT];
-#pragma unroll
-		for (int ib = 0; ib < IB; ib++) {
-			uint64_t i = ib0 + ib;
-			const uint64_t* src = nullptr;
-			if (i < i_end) src = a.in_ptrs ? a.in_ptrs[i] : a.in_base + i * a.in_stride;
-#pragma unroll
-			for (int j = 0; j < WPT / 2; j++) {
-				uint64_t w = w0 + 2 * tid + 512 * j;
-				uint64_t v0 = 0, v1 = 0;
-				if (src) {
+		for (int j = 0; j < WPT / 2; j++) {
+			const uint64_t w = w0 + 2 * tid + 512 * j;
+			uint64_t v0 = 0, v1 = 0;
+			if (i < i_end) {
+				if (fast) {
+					const ulonglong2 v = gload2(a.in_base + i * a.in_stride + w);
+					v0 = v.x;
+					v1 = v.y;
+				} else {
+					const uint64_t* src = a.in_ptrs ? a.in_ptrs[i] : a.in_base + i * a.in_stride;
 					if (w + 1 < a.W && ((reinterpret_cast<uintptr_t>(src) & 15u) == 0)) {
 						const ulonglong2 v = gload2(src + w);
 						v0 = v.x;
@@ -128,10 +134,42 @@ __device__ __forceinline__ void mac_tab_tile(const MacArgs& a, uint4* tab, uint6
 						if (w + 1 < a.W) v1 = gload(src + w + 1);
 					}
 				}
-				d[ib][2 * j] = make_uint2((uint32_t)v0, (uint32_t)(v0 >> 32));
-				d[ib][2 * j + 1] = make_uint2((uint32_t)v1, (uint32_t)(v1 >> 32));
 			}
+			d[ib][2 * j] = make_uint2((uint32_t)v0, (uint32_t)(v0 >> 32));
+			d[ib][2 * j + 1] = make_uint2((uint32_t)v1, (uint32_t)(v1 >> 32));
 		}
+	}
+}
+
+template <int RT, int WPT, int IB>
+__device__ __forceinline__ void mac_tab_tile(const MacArgs& a, uint4* tab, uint64_t w0,
+                                             uint64_t r0, uint64_t i_begin, uint64_t i_end,
+                                             uint64_t g, int out_mode) {
+	static_assert(RT % 2 == 0 && RT >= 2 && RT <= 16, "RT");
+	static_assert(WPT % 2 == 0 && WPT >= 2 && WPT <= 8, "WPT");
+	constexpr int NP = RT / 2;  // output pairs
+	static_assert(IB * NP * 16 <= 256, "one table entry per thread");
+	// tab[((ib*NP + p)*16 + k)*16 + n] = (c_{2p} * n x^{4k}, c_{2p+1} * n x^{4k})
+#define TAB(ib, p, k, n) tab[(((ib) * NP + (p)) * 16 + (k)) * 16 + (n)]
+	const int tid = threadIdx.x;
+	const bool fast = !a.in_ptrs && ((reinterpret_cast<uintptr_t>(a.in_base) & 15u) == 0) &&
+	                  (a.in_stride & 1) == 0 && w0 + 256ull * WPT <= a.W;
+
+	// word positions owned by this thread: pairs (2*tid + 512*j, +1);
+	// acc[j][p] = (lo, hi) of output 2p and (lo, hi) of output 2p+1
+	uint4 acc[WPT][NP];
+#pragma unroll
+	for (int j = 0; j < WPT; j++)
+#pragma unroll
+		for (int p = 0; p < NP; p++) acc[j][p] = make_uint4(0, 0, 0, 0);
+
+	// Software pipeline: the loads of batch b+1 are issued before the lookups
+	// of batch b, so their HBM latency is hidden behind LDS work; the table
+	// build of a batch then never waits for global memory.
+	uint2 d[IB][WPT];
+	uint64_t c0, c1;
+	mac_tab_load<RT, WPT, IB>(a, i_begin, i_end, w0, r0, fast, d, c0, c1);
+	for (uint64_t ib0 = i_begin; ib0 < i_end; ib0 += IB) {
 		// ---- build the split tables for IB inputs x RT outputs.  Thread
 		// (ib, p, n) owns entry n of every position k of its table pair:
 		// entry(k, n) = n(x) * c * x^{4k} = x^4 * entry(k-1, n), so it starts from
@@ -139,17 +177,10 @@ __device__ __forceinline__ void mac_tab_tile(const MacArgs& a, uint4* tab, uint6
 		// of a group write 16 consecutive entries of one 256-byte row, i.e. all
 		// 64 banks once: the build is free of LDS bank conflicts.
 		__syncthreads();  // previous batch finished reading tab
-		for (int t = tid; t < IB * NP * 16; t += 256) {
-			const int n = t & 15;
-			const int p = (t >> 4) % NP;
-			const int ib = (t >> 4) / NP;
-			const uint64_t i = ib0 + ib;
-			uint64_t c0 = 0, c1 = 0;
-			if (i < i_end) {
-				const uint64_t ra = r0 + 2 * p, rb = ra + 1;
-				if (ra < a.R) c0 = gload(a.coeff + ra * a.coeff_ld + i);
-				if (rb < a.R) c1 = gload(a.coeff + rb * a.coeff_ld + i);
-			}
+		if (tid < IB * NP * 16) {
+			const int n = tid & 15;
+			const int p = (tid >> 4) % NP;
+			const int ib = (tid >> 4) / NP;
 			uint64_t e0 = gf64_mul_nibble(c0, (unsigned)n);
 			uint64_t e1 = gf64_mul_nibble(c1, (unsigned)n);
 #pragma unroll
@@ -167,6 +198,10 @@ __device__ __forceinline__ void mac_tab_tile(const MacArgs& a, uint4* tab, uint6
 			}
 		}
 		__syncthreads();
+		// ---- next batch's loads, in flight during this batch's lookups
+		uint2 dn[IB][WPT];
+		uint64_t n0, n1;
+		mac_tab_load<RT, WPT, IB>(a, ib0 + IB, i_end, w0, r0, fast, dn, n0, n1);
 		// ---- lookups: four nibble positions per step, folded with 3-input XORs
 #pragma unroll
 		for (int ib = 0; ib < IB; ib++) {
@@ -198,6 +233,12 @@ __device__ __forceinline__ void mac_tab_tile(const MacArgs& a, uint4* tab, uint6
 				}
 			}
 		}
+#pragma unroll
+		for (int ib = 0; ib < IB; ib++)
+#pragma unroll
+			for (int j = 0; j < WPT; j++) d[ib][j] = dn[ib][j];
+		c0 = n0;
+		c1 = n1;
 	}
 #undef TAB
 
