@@ -286,8 +286,17 @@ __device__ __forceinline__ void mac_tab_tile(const MacArgs& a, uint4* tab, uint6
 	}
 }
 
+// Occupancy asked of the register allocator: 3 waves per SIMD where the LDS
+// holds 3 work-groups per CU (one wave per SIMD each; 160 KiB of LDS per CU),
+// otherwise no constraint (LDS already limits residency to 1-2 work-groups).
+template <int RT, int IB>
+struct TabWaves {
+	static constexpr int by_lds = 160 / (IB * RT * 2);
+	static constexpr int value = by_lds >= 3 ? 3 : 1;
+};
+
 template <int RT, int WPT, int IB>
-__global__ __launch_bounds__(256) void k_mac_tab(MacArgs a) {
+__global__ __launch_bounds__(256, (TabWaves<RT, IB>::value)) void k_mac_tab(MacArgs a) {
 	__shared__ uint4 tab[IB * (RT / 2) * 256];
 	const uint64_t g = blockIdx.y;
 	const uint64_t i_begin = g * a.in_per_group;
