@@ -263,6 +263,7 @@ static void prof_collect() {
 // Launch-shape override for tuning (parhip_tune_override); 0 fields = automatic.
 static thread_local int t_tune_rt = 0, t_tune_wpt = 0, t_tune_ib = 0;
 static thread_local int64_t t_tune_groups = 0;
+int64_t tune_groups_override() { return t_tune_groups; }
 
 Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev) {
 	Plan p;

@@ -79,6 +79,7 @@ static Plan16 plan16(uint64_t N, uint64_t R, uint64_t W) {
 	const uint64_t target = (uint64_t)device_cus(current_device()) * 4;
 	uint64_t groups = 1, base = tiles * rtiles;
 	if (base < target) groups = (target + base - 1) / base;
+	if (tune_groups_override() > 0) groups = (uint64_t)tune_groups_override();
 	groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, N / (4 * ib)));
 	groups = std::min<uint64_t>(groups, 65535);
 	uint64_t per = (N + groups - 1) / groups;

@@ -18,6 +18,8 @@ int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 int current_device();
 int current_method();
+// Split-K group count forced by parhip_tune_override (0 = automatic).
+int64_t tune_groups_override();
 int device_cus(int dev);
 int host_stream(hipStream_t* out);
 int dev_alloc(void** p, size_t bytes, hipStream_t s);

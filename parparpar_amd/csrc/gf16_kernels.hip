@@ -41,8 +41,48 @@ __global__ void k_gf16_coeffs(uint16_t* __restrict__ coeff, const uint16_t* __re
 	}
 }
 
+// Loads of one batch of IB inputs for k_gf16_tab: this thread's 4 words of
+// each input (zeros past the input range or the slice end) and, for thread
+// t < IB*4*NG, the coefficients of output pair t % (4*NG) of input t / (4*NG)
+// (build phase 1).  Coefficients first, so a wait for them leaves the data
+// loads in flight.
+template <int NG, int IB>
+__device__ __forceinline__ void gf16_tab_load(const Gf16Args& a, uint64_t ib0, uint64_t i_end,
+                                              uint64_t w0, uint64_t r0, uint2 (&d)[IB],
+                                              uint32_t& c0, uint32_t& c1) {
+	const int tid = threadIdx.x;
+	c0 = 0;
+	c1 = 0;
+	if (tid < IB * 4 * NG) {
+		const int pr = tid % (4 * NG);
+		const int ib = tid / (4 * NG);
+		const uint64_t i = ib0 + ib, e0 = r0 + 2 * pr;
+		if (i < i_end) {
+			if (e0 < a.R) c0 = a.coeff[e0 * a.coeff_ld + i];
+			if (e0 + 1 < a.R) c1 = a.coeff[(e0 + 1) * a.coeff_ld + i];
+		}
+	}
+#pragma unroll
+	for (int ib = 0; ib < IB; ib++) {
+		const uint64_t i = ib0 + ib;
+		uint64_t v = 0;
+		if (i < i_end) {
+			const uint16_t* src = a.in_base + i * a.in_stride;
+			if (w0 + 3 < a.W && ((reinterpret_cast<uintptr_t>(src) & 7u) == 0)) {
+				v = *reinterpret_cast<const uint64_t*>(src + w0);
+			} else {
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+					if (w0 + u < a.W) v |= (uint64_t)src[w0 + u] << (16 * u);
+			}
+		}
+		d[ib] = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+	}
+}
+
 template <int NG, int IB>
 __global__ __launch_bounds__(256) void k_gf16_tab(Gf16Args a) {
+	static_assert(IB * 4 * NG <= 256, "one basis pair per thread");
 	constexpr int TILE = 256 * 4;  // words per work-group
 	__shared__ uint4 tab[IB][NG][4][16];
 	// [input][j][output pair]; rows padded by 16 B so the 4 position rows a wave
@@ -64,43 +104,25 @@ __global__ __launch_bounds__(256) void k_gf16_tab(Gf16Args a) {
 #pragma unroll
 		for (int q = 0; q < NG; q++) acc[j][q] = make_uint4(0, 0, 0, 0);
 
+	// Software pipeline: batch b+1's loads are issued before batch b's lookups.
+	uint2 d[IB];
+	uint32_t c0, c1;
+	gf16_tab_load<NG, IB>(a, i_begin, i_end, w0, r0, d, c0, c1);
 	for (uint64_t ib0 = i_begin; ib0 < i_end; ib0 += IB) {
-		uint2 d[IB];
-#pragma unroll
-		for (int ib = 0; ib < IB; ib++) {
-			const uint64_t i = ib0 + ib;
-			uint64_t v = 0;
-			if (i < i_end) {
-				const uint16_t* src = a.in_base + i * a.in_stride;
-				if (w0 + 3 < a.W && ((reinterpret_cast<uintptr_t>(src) & 7u) == 0)) {
-					v = *reinterpret_cast<const uint64_t*>(src + w0);
-				} else {
-#pragma unroll
-					for (int u = 0; u < 4; u++)
-						if (w0 + u < a.W) v |= (uint64_t)src[w0 + u] << (16 * u);
-				}
-			}
-			d[ib] = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
-		}
 		__syncthreads();  // previous batch finished reading tab / bas
 		// build, phase 1: basis c_e * x^j (j < 16) of every (input, output) of
 		// the batch, an output pair per dword: bas[ib][j][pair].  One thread per
 		// (input, pair) writes whole dwords; the lanes of a store cover
 		// consecutive dwords of one row (no bank conflicts).
-		for (int t = tid; t < IB * 4 * NG; t += 256) {
-			const int pr = t % (4 * NG);
-			const int ib = t / (4 * NG);
-			const uint64_t i = ib0 + ib, e0 = r0 + 2 * pr;
-			uint32_t c0 = 0, c1 = 0;
-			if (i < i_end) {
-				if (e0 < a.R) c0 = a.coeff[e0 * a.coeff_ld + i];
-				if (e0 + 1 < a.R) c1 = a.coeff[(e0 + 1) * a.coeff_ld + i];
-			}
+		if (tid < IB * 4 * NG) {
+			const int pr = tid % (4 * NG);
+			const int ib = tid / (4 * NG);
+			uint32_t b0 = c0, b1 = c1;
 #pragma unroll
 			for (int j = 0; j < 16; j++) {
-				bas[ib][j][pr] = c0 | (c1 << 16);
-				c0 = gf16_mulx(c0);
-				c1 = gf16_mulx(c1);
+				bas[ib][j][pr] = b0 | (b1 << 16);
+				b0 = gf16_mulx(b0);
+				b1 = gf16_mulx(b1);
 			}
 		}
 		__syncthreads();
@@ -126,6 +148,10 @@ __global__ __launch_bounds__(256) void k_gf16_tab(Gf16Args a) {
 			tab[ib][q][k][n] = ent;
 		}
 		__syncthreads();
+		// next batch's loads, in flight during this batch's lookups
+		uint2 dn[IB];
+		uint32_t n0c, n1c;
+		gf16_tab_load<NG, IB>(a, ib0 + IB, i_end, w0, r0, dn, n0c, n1c);
 #pragma unroll
 		for (int ib = 0; ib < IB; ib++) {
 #pragma unroll
@@ -148,6 +174,10 @@ __global__ __launch_bounds__(256) void k_gf16_tab(Gf16Args a) {
 				}
 			}
 		}
+#pragma unroll
+		for (int ib = 0; ib < IB; ib++) d[ib] = dn[ib];
+		c0 = n0c;
+		c1 = n1c;
 	}
 
 	// write: output e = r0 + 8q + m, words w0..w0+3
