@@ -95,6 +95,12 @@ def test_with_coeff_and_matrix(gpu, oracle):
     assert np.array_equal(m.reshape(r, n), oracle.cauchy(n, r, 3, 500))
     big = binding.build_coefficient_matrix(1000, 40, 0, 1000)
     assert np.array_equal(big.reshape(40, 1000), oracle.cauchy(1000, 40, 0, 1000))
+    # larger matrices: wide, narrow (rows shorter than a wave) and ragged
+    # shapes, and colliding points (0 -> 1)
+    for (bn, br, bfi, bfr) in [(1000, 100, 0, 1000), (37, 2000, 5, 9), (65537, 1, 0, 65537),
+                               (3, 30000, 0, 1), (4099, 17, 100, 100)]:
+        got = binding.build_coefficient_matrix(bn, br, bfi, bfr).reshape(br, bn)
+        assert np.array_equal(got, oracle.cauchy(bn, br, bfi, bfr)), (bn, br, bfi, bfr)
     co = rand_words(rng, r, n)  # arbitrary coefficients, including structure-free ones
     co[0, :] = 0
     co[1, :] = 1
