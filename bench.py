@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", default="par3-create",
                     choices=["par3-create", "par2-create", "batch", "repair", "host"],
-                    help="par3-create (default line); the others are reported in DESIGN.md")
+                    help="par3-create (default line); the others are secondary workloads")
     ap.add_argument("--jobs", type=int, default=200)
     ap.add_argument("--rate", type=float, default=4.0, help="repair requests per second")
     ap.add_argument("--requests", type=int, default=60)
@@ -58,7 +58,7 @@ def parse():
 
 def cpu_baseline(args, d_inputs, budget_s):
     """Reference CPU engine on a bounded sample (first S slices) + bit-exact check
-    of the GPU engine on the same sample.  Uses the checker only here."""
+    of the GPU engine on the same sample.  The only place bench.py uses the oracle."""
     import numpy as np
     import torch
     from oracle import oracle as O
@@ -115,8 +115,8 @@ def cpu_baseline(args, d_inputs, budget_s):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every GPU box here has one GPU; "
-              "--gpus %d is not supported (the path would shard slices by word range)" % args.gpus,
+        print("bench.py measures a single GPU; --gpus %d is not supported "
+              "(the path would shard slices by word range, one range per GPU)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "par2-create":
@@ -132,8 +132,8 @@ def main():
 
 def main_par3(args):
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every GPU box here has one GPU; "
-              "--gpus %d is not supported (the path would shard slices by word range)" % args.gpus,
+        print("bench.py measures a single GPU; --gpus %d is not supported "
+              "(the path would shard slices by word range, one range per GPU)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     import torch

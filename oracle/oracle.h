@@ -2,8 +2,8 @@
  * oracle.h -- CPU restatement of the reference's recovery-slice arithmetic.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (parparpar_amd/, include/)
- * includes, links or calls this code.  Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg load it, and only as the checker.
+ * includes, links or calls this code.  Only tests/, the smoke entry point and
+ * bench.py's cpu_baseline leg load it, and only to check results.
  *
  * Every function cites the reference file:line it restates.  The pins that
  * check this restatement (golden vectors, KATs, independent methods, brute

@@ -1,8 +1,8 @@
 """ctypes front of the oracle (CPU restatement) and of the reference engine
 compiled in place under oracle/_ref.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker, never as the thing
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke entry point and the
+cpu_baseline leg of bench.py, always to check results, never as the thing
 measured or shipped.  The product package never imports this module.
 """
 import ctypes

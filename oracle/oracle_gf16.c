@@ -1,7 +1,7 @@
 /*
  * oracle_gf16.c -- CPU restatement of the reference's PAR2 GF(2^16) muladd.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  The product never links this file.
+ * TEST INFRASTRUCTURE ONLY (the oracle).  The product never links this file.
  *
  * Field: GF(2)[x] / 0x1100B, generator 2 (gf16/gf16_global.h:9).
  * Words: 16-bit little-endian; an odd trailing byte is a word whose high byte

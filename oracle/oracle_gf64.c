@@ -1,7 +1,7 @@
 /*
  * oracle_gf64.c -- CPU restatement of the reference's GF(2^64) recovery path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  The product never links this file.
+ * TEST INFRASTRUCTURE ONLY (the oracle).  The product never links this file.
  *
  * Field: GF(2)[x] / (x^64 + x^4 + x^3 + x + 1)   (gf64/gf64_global.h:8)
  * Words: slices are arrays of native little-endian uint64 words.

@@ -67,7 +67,7 @@ def test_version_without_gpu(built_lib):
 
 
 def test_library_links_no_oracle(built_lib):
-    # the product must not depend on the checker
+    # the product must not depend on the oracle
     out = subprocess.run(["ldd", built_lib], stdout=subprocess.PIPE, text=True).stdout
     assert "oracle" not in out and "libref" not in out
     srcs = glob.glob(os.path.join(ROOT, "parparpar_amd", "**", "*.*"), recursive=True)

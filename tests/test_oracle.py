@@ -1,4 +1,4 @@
-"""Pins of the oracle (CPU restatement) before it is trusted as the checker.
+"""Pins of the oracle (CPU restatement) before its results are trusted.
 
 Each oracle function is checked against the reference's own golden vectors and
 known answers, against an independent method (the inverse and the product are
