@@ -17,7 +17,8 @@ generator matching bit for bit.
 
 and, in the same way, the grids of test/par3-small-r-fused-parity.js,
 par3-2d-decomposition-parity.js, par3-barycentric-parity.js,
-par3-accumulate-binding-parity.js and par3-full-recovery-parity.js.
+par3-accumulate-binding-parity.js, par3-full-recovery-parity.js and
+par3-repair-parity.js.
 """
 import numpy as np
 import pytest
@@ -319,3 +320,46 @@ def test_repair_native_matrix_flow(gpu, oracle, N, n, bs):
                                         accumulate=True)
     assert binding.solve_and_reconstruct(A, rhs, n, bs) is True
     assert np.array_equal(rhs, data[missing])
+
+
+# test/par3-repair-parity.js:427-433: (totalInput, recoveryCount, blockSize,
+# missingCounts), firstInput [0, 100], firstRecovery [200, 300]
+REPAIR = [(8, 3, 64, [1, 2, 3]), (16, 4, 128, [1, 2, 3, 4]), (8, 2, 1024, [1, 2]),
+          (32, 5, 256, [1, 2, 3, 4, 5]), (10, 3, 512, [1, 2, 3])]
+
+
+@pytest.mark.gpu
+def test_repair_parity_scenarios(gpu, oracle):
+    """Every scenario and missing count of the reference's repair parity test,
+    repaired twice: through the host entries (coefficient matrix, RHS by
+    compute_recovery_with_coeff, solve_and_reconstruct), and on the device
+    (parhip_d_gf64_repair); both give back the lost slices, and the host solve
+    equals the oracle's Gauss-Jordan."""
+    import torch
+    from parparpar_amd import binding, repair
+    nxt = mulberry32(0x4E9A1)
+    for (N, R, bs, counts) in REPAIR:
+        fi = (nxt() * 101) >> 32
+        fr = ((nxt() * 101) >> 32) + 200
+        data = words(nxt, N * bs // 8).reshape(N, bs // 8)
+        rec = np.zeros((R, bs // 8), dtype=np.uint64)
+        binding.compute_recovery(data, rec, N, R, bs, fi, fr)
+        M = binding.build_coefficient_matrix(N, R, fi, fr).reshape(R, N)
+        d_data = torch.from_numpy(data.view(np.int64)).cuda()
+        d_rec = torch.from_numpy(rec.view(np.int64)).cuda()
+        for k in counts:
+            order = np.argsort(np.array([nxt() for _ in range(N)], dtype=np.uint64), kind="stable")
+            missing = np.sort(order[:k])  # k distinct positions
+            known = np.setdiff1d(np.arange(N), missing)
+            A = np.ascontiguousarray(M[:k][:, missing])
+            rhs = rec[:k].copy()
+            binding.compute_recovery_with_coeff(np.ascontiguousarray(M[:k][:, known]),
+                                                np.ascontiguousarray(data[known]), rhs, len(known), k, bs,
+                                                accumulate=True)
+            rc, A_o, x_o = oracle.solve_and_reconstruct(A.copy(), rhs.copy())
+            assert rc == 0
+            assert binding.solve_and_reconstruct(A, rhs, k, bs) is True
+            assert np.array_equal(rhs, x_o) and np.array_equal(rhs, data[missing]), (N, R, bs, k)
+            out = repair.repair(d_data, d_rec, missing, fi, fr)
+            torch.cuda.synchronize()
+            assert np.array_equal(out.cpu().numpy().view(np.uint64), data[missing]), (N, R, bs, k)
