@@ -210,7 +210,32 @@ def gf64_solve(A, b, n):
     return x
 
 
+def _check_words(buf, count, what):
+    if count > buf.nbytes // 8:
+        raise ValueError("%s buffer too small for %d words" % (what, count))
+
+
+def _check_region(out, inBuf, coeff, len_words, n_coeff):
+    # MulArr_NAPI / Gf64Encoder.mul_arr rules (gf64_addon.cc:3087-3106, :408-416):
+    # non-negative counts, n_coeff > 0 when len_words > 0, every count within
+    # its buffer (compared in words, so huge counts cannot wrap)
+    if len_words < 0 or n_coeff < 0:
+        raise ValueError("len_words and n_coeff must be non-negative")
+    if len_words > 0 and n_coeff == 0:
+        raise ValueError("n_coeff must be positive when len_words > 0")
+    _check_words(out, len_words, "out")
+    _check_words(inBuf, len_words, "in")
+    _check_words(coeff, n_coeff, "coeff")
+
+
+def _check_count(v, name):
+    # Gf64Encoder's K / G bound (gf64_addon.cc:497-498, :655-656, :878-890)
+    if v < 0 or v > 65536:
+        raise ValueError("%s out of range [0, 65536]" % name)
+
+
 def mul_arr(out, inBuf, coeff, len_words, n_coeff):
+    _check_region(out, inBuf, coeff, len_words, n_coeff)
     check(lib().parhip_gf64_region_mul_arr(_buf(out, "out"), _buf(inBuf, "in"), _buf(coeff, "coeff"),
                                            len_words, n_coeff))
     return out
@@ -225,6 +250,10 @@ class Gf64Encoder:
     """Region operations (gf64/gf64_global.h:23-29) as the addon's Gf64Encoder exposes them."""
 
     def mul(self, out, inBuf, constant, len_words):
+        if len_words < 0:
+            raise ValueError("Length must be non-negative")
+        _check_words(out, len_words, "out")
+        _check_words(inBuf, len_words, "in")
         check(lib().parhip_gf64_region_mul(_buf(out, "out"), _buf(inBuf, "in"), len_words, constant))
         return out
 
@@ -232,11 +261,20 @@ class Gf64Encoder:
         return mul_arr(out, inBuf, coeff, len_words, n_coeff)
 
     def muladd_arr(self, out, inBuf, coeff, len_words, n_coeff):
+        _check_region(out, inBuf, coeff, len_words, n_coeff)
         check(lib().parhip_gf64_region_muladd_arr(_buf(out, "out"), _buf(inBuf, "in"),
                                                   _buf(coeff, "coeff"), len_words, n_coeff))
         return out
 
     def coupled_muladd_arr(self, out, in_blocks, coeff_blocks, len_words):
+        _check_count(len(in_blocks), "G")
+        if len_words < 0:
+            raise ValueError("Length must be non-negative")
+        _check_words(out, len_words, "out")
+        _check_words(coeff_blocks, len(in_blocks), "coeff")
+        for b in in_blocks:
+            _buf(b, "in_blocks element")
+            _check_words(b, len_words, "in_blocks element")
         ptrs = _ptr_array(in_blocks)
         check(lib().parhip_gf64_region_coupled_muladd_arr(
             _buf(out, "out"), ctypes.addressof(ptrs), _buf(coeff_blocks, "coeff"), len_words,
@@ -245,7 +283,16 @@ class Gf64Encoder:
 
     def fused_output_muladd_arr(self, outs, inBuf, coeffs, len_words):
         """outs[k] ^= in * coeffs[k] (coeffs: uint64 array; one pointer per output)."""
+        _check_count(len(outs), "K")
+        if len_words < 0:
+            raise ValueError("Length must be non-negative")
+        _check_words(inBuf, len_words, "in")
         coeffs = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        if coeffs.size < len(outs):
+            raise ValueError("Coefficients array length must equal K")
+        for o in outs:
+            _buf(o, "outs element")
+            _check_words(o, len_words, "outs element")
         optrs = _ptr_array(outs)
         cptrs = (ctypes.c_void_p * len(outs))(*[coeffs.ctypes.data + 8 * k for k in range(len(outs))])
         check(lib().parhip_gf64_region_fused_output_muladd_arr(
@@ -253,7 +300,22 @@ class Gf64Encoder:
         return outs
 
     def two_d_muladd_arr(self, outs, in_blocks, coeff2d, K_stride, len_words):
+        _check_count(len(outs), "K")
+        _check_count(len(in_blocks), "G")
+        if len_words < 0:
+            raise ValueError("len must be non-negative")
+        if K_stride < len(in_blocks):
+            raise ValueError("K_stride must be >= G")
+        _check_count(K_stride, "K_stride")
         coeff2d = np.ascontiguousarray(coeff2d, dtype=np.uint64)
+        if coeff2d.size < len(outs) * K_stride:
+            raise ValueError("coeff_block_2d buffer too small for K * K_stride")
+        for o in outs:
+            _buf(o, "outs element")
+            _check_words(o, len_words, "outs element")
+        for b in in_blocks:
+            _buf(b, "in_blocks element")
+            _check_words(b, len_words, "in_blocks element")
         optrs = _ptr_array(outs)
         iptrs = _ptr_array(in_blocks)
         check(lib().parhip_gf64_region_2d_muladd_arr(

@@ -363,3 +363,38 @@ def test_repair_parity_scenarios(gpu, oracle):
             out = repair.repair(d_data, d_rec, missing, fi, fr)
             torch.cuda.synchronize()
             assert np.array_equal(out.cpu().numpy().view(np.uint64), data[missing]), (N, R, bs, k)
+
+
+@pytest.mark.gpu
+def test_mul_arr_cells_and_guards(gpu, oracle):
+    """test/par3-gf64-mularr-parity.js: n_coeff {2, 4, 8, 16} x len {8, 64, 256,
+    1024} bytes x 10 trials of the cycling mul_arr, then its guards (:217-233):
+    n_coeff = 0 with len > 0 and counts of 2^61 against 64-byte buffers are
+    rejected on both entries; len = n_coeff = 0 stays a no-op."""
+    from parparpar_amd import binding
+    nxt = mulberry32(0x3A11)
+    enc = binding.Gf64Encoder()
+    for n_coeff in (2, 4, 8, 16):
+        for nbytes in (8, 64, 256, 1024):
+            for _ in range(10):
+                inp = words(nxt, nbytes // 8)
+                co = words(nxt, n_coeff)
+                out = np.zeros_like(inp)
+                enc.mul_arr(out, inp, co, nbytes // 8, n_coeff)
+                assert np.array_equal(out, _mul_arr_expected(oracle, inp, co)), (n_coeff, nbytes)
+    g = np.zeros(8, dtype=np.uint64)
+    for fn in (binding.mul_arr, enc.mul_arr):
+        for args in ((8, 0), (2 ** 61, 1), (1, 2 ** 61), (-1, 1)):
+            with pytest.raises(ValueError):
+                fn(g, g.copy(), g.copy(), *args)
+        fn(g, g.copy(), g.copy(), 0, 0)
+    with pytest.raises(ValueError):
+        enc.muladd_arr(g, g.copy(), g.copy(), 9, 1)
+    with pytest.raises(ValueError):
+        enc.mul(g, g.copy(), 3, 9)
+    with pytest.raises(ValueError):
+        enc.coupled_muladd_arr(g, [g.copy()] * 2, g[:1].copy(), 8)  # coeff shorter than G
+    with pytest.raises(ValueError):
+        enc.fused_output_muladd_arr([g, np.zeros(4, dtype=np.uint64)], g.copy(), [1, 2], 8)
+    with pytest.raises(ValueError):
+        enc.two_d_muladd_arr([g], [g.copy(), g.copy()], np.ones(2, dtype=np.uint64), 1, 8)  # K_stride < G
