@@ -17,7 +17,8 @@ generator matching bit for bit.
 
 and, in the same way, the grids of test/par3-small-r-fused-parity.js,
 par3-2d-decomposition-parity.js, par3-barycentric-parity.js,
-par3-accumulate-binding-parity.js, par3-full-recovery-parity.js and
+par3-accumulate-binding-parity.js, par3-full-recovery-parity.js,
+par3-input-parallel-parity.js, par3-gf64-mularr-parity.js and
 par3-repair-parity.js.
 """
 import numpy as np
@@ -398,3 +399,20 @@ def test_mul_arr_cells_and_guards(gpu, oracle):
         enc.fused_output_muladd_arr([g, np.zeros(4, dtype=np.uint64)], g.copy(), [1, 2], 8)
     with pytest.raises(ValueError):
         enc.two_d_muladd_arr([g], [g.copy(), g.copy()], np.ones(2, dtype=np.uint64), 1, 8)  # K_stride < G
+
+
+@pytest.mark.gpu
+def test_input_parallel_cases(gpu, oracle):
+    """test/par3-input-parallel-parity.js:57-62: the shapes the reference splits
+    over the input domain, at its thread counts; identical bytes, equal to the
+    oracle."""
+    from parparpar_amd import binding
+    nxt = mulberry32(0x7800E3D)
+    for (N, R, B, threads) in [(64, 2, 512, (1, 2, 4, 8, 16)), (256, 4, 1024, (1, 2, 8, 16)),
+                               (512, 8, 4096, (1, 8, 12, 16)), (1024, 8, 8192, (1, 8, 16))]:
+        inp = words(nxt, N * B // 8).reshape(N, B // 8)
+        want = oracle.recovery(inp, R, 0, N)
+        for t in threads:
+            out = np.zeros((R, B // 8), dtype=np.uint64)
+            binding.compute_recovery_full(inp, out, N, R, B, 0, N, t, False)
+            assert np.array_equal(out, want), (N, R, B, t)
