@@ -54,13 +54,15 @@ int parhip_version(void);
 const char *parhip_last_error(void);
 /* Select the HIP device used by calls from this thread (default 0). */
 int parhip_set_device(int device);
-/* Force a contraction strategy for this thread (PARHIP_METHOD_*). The MFMA
- * and 8-bit table strategies apply to strided slices; pointer-array calls use
- * the nibble tables. */
+/* Force a contraction strategy for this thread (PARHIP_METHOD_*), for the
+ * GF(2^64) contraction and (MFMA, MFMA_I8) the PAR2 GF(2^16) one.  The MFMA and
+ * 8-bit table strategies apply to strided slices; pointer-array calls use the
+ * nibble tables. */
 int parhip_set_method(int method);
-/* Force the table-contraction launch shape for this thread (tuning; no
- * counterpart in the reference). rt = outputs per work-group, wpt = words per
- * thread, ib = inputs per LDS batch, groups = split-K rows; 0 = automatic. */
+/* Force the contraction launch shape for this thread (tuning; no counterpart
+ * in the reference). Tables: rt = outputs per work-group, wpt = words per
+ * thread, ib = inputs per LDS batch; MFMA: rt / wpt = the kernel's two tile
+ * dimensions; groups = split-K rows; 0 = automatic. */
 int parhip_tune_override(int rt, int wpt, int ib, int64_t groups);
 /* Release cached device memory (coefficient cache, workspaces, this thread's
  * staging ring). */

@@ -264,6 +264,11 @@ static void prof_collect() {
 static thread_local int t_tune_rt = 0, t_tune_wpt = 0, t_tune_ib = 0;
 static thread_local int64_t t_tune_groups = 0;
 int64_t tune_groups_override() { return t_tune_groups; }
+void tune_shape_override(int* rt, int* wpt, int* ib) {
+	*rt = t_tune_rt;
+	*wpt = t_tune_wpt;
+	*ib = t_tune_ib;
+}
 
 Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev) {
 	Plan p;
@@ -527,6 +532,8 @@ int parhip_prof_get(const char* kernel, double* total_ms, uint64_t* launches) {
 int parhip_tune_override(int rt, int wpt, int ib, int64_t groups) {
 	if ((rt || wpt || ib) && !mac_tab_supported(rt ? rt : 8, wpt ? wpt : 4, ib ? ib : 4) &&
 	    !mac_tab8_supported(rt ? rt : 4, wpt ? wpt : 4, ib ? ib : 1) &&
+	    !gf16_mfma_supported(rt ? rt : 4, wpt ? wpt : 4, true) &&
+	    !gf16_mfma_supported(rt ? rt : 4, wpt ? wpt : 2, false) &&
 	    !mfma_supported(rt ? rt : 4, wpt ? wpt : 2, ib ? ib : 2, true) &&
 	    !mfma_supported(rt ? rt : 4, wpt ? wpt : 2, ib ? ib : 2, false))
 		return fail(PARHIP_ERR_ARG, "tune_override: unsupported (rt=%d wpt=%d ib=%d)", rt, wpt, ib);

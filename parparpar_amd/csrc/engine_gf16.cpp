@@ -89,8 +89,71 @@ static Plan16 plan16(uint64_t N, uint64_t R, uint64_t W) {
 	return p;
 }
 
+// GF(2) bit-matrix strategy on MFMA (gf16_mfma.hip): row words of every
+// coefficient, then the MFMA contraction; split-K partials folded as for the
+// tables.  Output tiles of 8 outputs and 4 column blocks of 32 words per wave
+// (measured best for both types at 32768 x 32 KiB, R = 512).
+static int run_gf16_mfma(const Gf16Args& m, bool fp4, hipStream_t s) {
+	int ot = 4, nb = 4, trt, twpt, tib;
+	tune_shape_override(&trt, &twpt, &tib);
+	if ((trt || twpt) && gf16_mfma_supported(trt ? trt : ot, twpt ? twpt : nb, fp4)) {
+		ot = trt ? trt : ot;
+		nb = twpt ? twpt : nb;
+	}
+	const uint64_t span = 4ull * 32 * nb;
+	const uint64_t base = ((m.W + span - 1) / span) * ((m.R + 2 * ot - 1) / (2 * ot));
+	const uint64_t target = (uint64_t)device_cus(current_device()) * 2;
+	uint64_t groups = base < target ? (target + base - 1) / base : 1;
+	if (tune_groups_override() > 0) groups = (uint64_t)tune_groups_override();
+	groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, m.N / 16));
+	// fp4 counts are fp32 sums of 0/1 products: exact while 16 * inputs < 2^24
+	if (fp4) groups = std::max<uint64_t>(groups, (m.N + (1u << 19) - 1) >> 19);
+	groups = std::min<uint64_t>(groups, 65535);
+	uint64_t per = (m.N + groups - 1) / groups;
+	per = (per + 3) / 4 * 4;  // stages of 4 inputs, pairs never straddle groups
+	groups = (m.N + per - 1) / per;
+	if (fp4 && per >= (1u << 20))
+		return fail(PARHIP_ERR_ARG, "gf16 mfma: %llu inputs per group exceed the exact fp32 count",
+		            (unsigned long long)per);
+	const size_t rows_bytes = (size_t)m.R * ((m.N + 1) / 2) * 16 * sizeof(uint32_t);
+	const size_t part_bytes =
+	    groups > 1 ? (size_t)gf16_reduce_scratch_words(groups, m.R, m.W) * sizeof(uint16_t) : 0;
+	void* ws = nullptr;
+	int rc = dev_alloc(&ws, rows_bytes + part_bytes, s);
+	if (rc) return rc;
+	hipError_t e;
+	{
+		ProfScope ps("gf16_rows", s);
+		e = launch_gf16_rows(m.coeff, m.coeff_ld, m.N, m.R, (uint32_t*)ws, s);
+	}
+	Gf16MfmaArgs a{};
+	a.in_base = m.in_base;
+	a.in_stride = m.in_stride;
+	a.out_base = m.out_base;
+	a.out_stride = m.out_stride;
+	a.rows = (const uint32_t*)ws;
+	a.N = m.N;
+	a.R = m.R;
+	a.W = m.W;
+	a.in_per_group = per;
+	a.partial = groups > 1 ? (uint16_t*)((char*)ws + rows_bytes) : nullptr;
+	a.accumulate = m.accumulate;
+	if (e == hipSuccess) {
+		ProfScope ps("gf16_mfma", s);
+		e = launch_gf16_mfma(a, groups, ot, nb, fp4, s);
+	}
+	if (e == hipSuccess && groups > 1) {
+		ProfScope ps("gf16_reduce", s);
+		e = launch_gf16_reduce(a.partial, groups, m.R, m.W, m.out_base, m.out_stride, m.accumulate, s);
+	}
+	dev_free(ws, s);
+	return e == hipSuccess ? PARHIP_OK : hip_fail(e, "k_gf16_mfma");
+}
+
 int run_gf16_contraction(Gf16Args a, hipStream_t s) {
 	if (a.N == 0 || a.R == 0 || a.W == 0) return PARHIP_OK;
+	if (current_method() == PARHIP_METHOD_MFMA || current_method() == PARHIP_METHOD_MFMA_I8)
+		return run_gf16_mfma(a, current_method() == PARHIP_METHOD_MFMA, s);
 	Plan16 p = plan16(a.N, a.R, a.W);
 	a.in_per_group = p.in_per_group;
 	a.partial = nullptr;

@@ -20,6 +20,8 @@ int current_device();
 int current_method();
 // Split-K group count forced by parhip_tune_override (0 = automatic).
 int64_t tune_groups_override();
+// Tile shape forced by parhip_tune_override (0 fields = automatic).
+void tune_shape_override(int* rt, int* wpt, int* ib);
 int device_cus(int dev);
 int host_stream(hipStream_t* out);
 int dev_alloc(void** p, size_t bytes, hipStream_t s);

@@ -1,0 +1,350 @@
+// gf16_mfma.hip -- PAR2 GF(2^16) contraction as a GF(2) bit-matrix product on MFMA.
+//
+// Multiplication by a coefficient c is a 16x16 matrix over GF(2): column j is
+// c * x^j (mod 0x1100B).  For one 16-bit word w,
+//     out_bits[(e, o)] = parity( sum_{(i, j)} M_{c(e,i)}[o][j] * bit_j(in_i[w]) ),
+// an integer matrix product (rows = 16 R, K = 16 N, columns = words) whose
+// parity is the GF(2) result -- the same scheme as gf64_mfma.hip with 16x16
+// blocks.  k_gf16_rows prepares the matrix rows of every coefficient once,
+// two inputs per 32-bit word: rows[(e*P + ip)*16 + o] = Row_o(c(e, 2ip)) |
+// Row_o(c(e, 2ip+1)) << 16, where bit j of Row_o(c) is bit o of c * x^j.
+// The main kernel stages them in LDS, expanded to 0/1 matrix elements:
+//   - fp4:  v_mfma_scale_f32_32x32x64_f8f6f4, e2m1 elements 0 / 0.5 with block
+//           scale 2 (products 0 or 1, the fp32 sum is an exact count);
+//           K = 64 = 4 inputs: lane L = row + 32h holds input pair h, its 32
+//           bits placed with the fp4 expansion of gf64_mfma.hip;
+//   - int8: v_mfma_i32_32x32x32_i8, K = 32 = 2 inputs: lane h holds one input's
+//           16 bits, one byte each.
+// The words are expanded the same way, so A and B use one K order.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "gf16_dev.h"
+#include "kernels.h"
+
+namespace parhip {
+
+namespace {
+typedef int g16_v4i __attribute__((ext_vector_type(4)));
+typedef int g16_v8i __attribute__((ext_vector_type(8)));
+typedef int g16_v16i __attribute__((ext_vector_type(16)));
+typedef float g16_v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint32_t g16_spread4_i8(uint32_t x) {  // 4 low bits -> 4 bytes of 0/1
+	return ((x & 0xFu) * 0x00204081u) & 0x01010101u;
+}
+
+__device__ __forceinline__ uint4 g16_expand_i8(uint32_t x) {  // 16 bits -> 16 bytes
+	return make_uint4(g16_spread4_i8(x), g16_spread4_i8(x >> 4), g16_spread4_i8(x >> 8),
+	                  g16_spread4_i8(x >> 12));
+}
+
+__device__ __forceinline__ uint4 g16_expand_fp4(uint32_t x) {  // bit v+4q -> nibble q of dword v
+	return make_uint4(x & 0x11111111u, (x >> 1) & 0x11111111u, (x >> 2) & 0x11111111u,
+	                  (x >> 3) & 0x11111111u);
+}
+}  // namespace
+
+// rows[(e*P + ip)*16 + o], P = ceil(N/2); inputs past N contribute zero rows.
+__global__ void k_gf16_rows(const uint16_t* __restrict__ coeff, uint64_t ld, uint64_t N, uint64_t R,
+                            uint32_t* __restrict__ rows) {
+	const uint64_t P = (N + 1) / 2;
+	const uint64_t total = R * P * 16;
+	for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+	     idx += (uint64_t)gridDim.x * blockDim.x) {
+		const uint64_t o = idx & 15, q = idx >> 4;
+		const uint64_t e = q / P, ip = q - e * P;
+		uint32_t word = 0;
+#pragma unroll
+		for (int half = 0; half < 2; half++) {
+			const uint64_t i = 2 * ip + half;
+			uint32_t p = i < N ? coeff[e * ld + i] : 0u;  // c * x^j, j = 0..15
+			uint32_t row = 0;
+#pragma unroll
+			for (int j = 0; j < 16; j++) {
+				row |= ((p >> o) & 1u) << j;
+				p = gf16_mulx(p);
+			}
+			word |= row << (16 * half);
+		}
+		rows[idx] = word;
+	}
+}
+
+// One work-group: 4 waves; lane `col` of a wave owns NB consecutive words
+// (wbase + NB*col + nb, one 32-wide column block per nb), so a lane reads them
+// with one NB*16-bit load per input and writes them with one store per output.
+// OT output tiles of 32 rows (2 outputs each); an input range that is a
+// multiple of 4.  Stages of 4*IB inputs: the next stage's row words and input
+// words are loaded while the current stage feeds MFMA, its A operands go to
+// the other half of a double-buffered LDS ring, one barrier per stage.  Loads
+// use clamped, in-bounds addresses: words past W and outputs past R only reach
+// accumulator columns / rows that are never stored, and inputs past the range
+// get zero rows when they are stored to LDS.
+template <int OT, int NB, int IB, bool FP4>
+__global__ __launch_bounds__(256, 1) void k_gf16_mfma(Gf16MfmaArgs a) {
+	constexpr int KS = FP4 ? 1 : 2;             // MFMA K steps per 4 inputs
+	constexpr int LPT = (IB * OT * 64 + 255) / 256;  // row words loaded per thread per stage
+	static_assert((IB * OT * 64) % 256 == 0, "whole waves stage row words");
+	static_assert(NB == 2 || NB == 4, "a lane's words fit one 32- or 64-bit access");
+	typedef typename std::conditional<NB == 4, uint64_t, uint32_t>::type xw_t;
+	__shared__ uint4 sA[2][IB][OT][KS][64];
+
+	const int tid = threadIdx.x;
+	const int lane = tid & 63;
+	const int wave = tid >> 6;
+	const int col = lane & 31;
+	const int h = lane >> 5;
+	// XCD-aware order (as in gf64_mfma.hip): runs of 8 * Z ids cover 8 word
+	// blocks x all Z output tiles, so the tiles that read the same words run on
+	// one XCD together.
+	const uint64_t Z = (a.R + 2 * OT - 1) / (2 * OT);
+	const uint64_t L = blockIdx.x;
+	const uint64_t rem = L % (8 * Z);
+	const uint64_t xblk = (L / (8 * Z)) * 8 + (rem & 7);
+	if (xblk * 4 * 32 * NB >= a.W) return;  // padding of the last run (whole work-group)
+	const uint64_t wbase = (xblk * 4 + wave) * (32ull * NB);
+	const uint64_t w0 = wbase + (uint64_t)NB * col;  // this lane's first word
+	const uint64_t r0 = (rem >> 3) * (2 * OT);
+	const uint64_t g = blockIdx.y;
+	const uint64_t i_begin = g * a.in_per_group;
+	uint64_t i_end = i_begin + a.in_per_group;
+	if (i_end > a.N) i_end = a.N;
+	const uint64_t P = (a.N + 1) / 2;
+	// whole NB-word accesses: aligned rows and a lane range inside the slice
+	const bool vec_in = ((reinterpret_cast<uintptr_t>(a.in_base) & (2 * NB - 1)) == 0) &&
+	                    (a.in_stride % NB) == 0 && w0 + NB <= a.W;
+
+	typedef typename std::conditional<FP4, g16_v16f, g16_v16i>::type acc_t;
+	acc_t acc[OT][NB];
+#pragma unroll
+	for (int t = 0; t < OT; t++)
+#pragma unroll
+		for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+			for (int e = 0; e < 16; e++) acc[t][nb][e] = 0;
+
+	uint32_t pre[LPT];      // row words of the next stage
+	xw_t xp[4 * IB];        // input words of the next stage (NB words per input)
+	xw_t xc[4 * IB];        // input words of the current stage
+	const uint64_t i_last = i_end - 1;
+
+	auto load_stage = [&](uint64_t ibase) {
+#pragma unroll
+		for (int k = 0; k < LPT; k++) {
+			const int j = tid + 256 * k;  // (ib, t, pair, row)
+			const int ib = j / (OT * 64), t = (j >> 6) % OT, rho = j & 31, pr = (j >> 5) & 1;
+			uint64_t e = r0 + 2 * t + (rho >> 4);
+			e = e < a.R ? e : a.R - 1;
+			uint64_t ip = (ibase + 4 * ib) / 2 + pr;
+			ip = ip < P ? ip : P - 1;
+			pre[k] = a.rows[(e * P + ip) * 16 + (rho & 15)];
+		}
+#pragma unroll
+		for (int u = 0; u < 4 * IB; u++) {
+			const uint64_t i = ibase + u;
+			const uint64_t ic = i < i_end ? i : i_last;
+			const uint16_t* src = a.in_base + ic * a.in_stride;
+			if (vec_in) {
+				xp[u] = *reinterpret_cast<const xw_t*>(src + w0);
+			} else {
+				xw_t v = 0;
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++) {
+					const uint64_t w = w0 + nb;
+					v |= (xw_t)src[w < a.W ? w : a.W - 1] << (16 * nb);
+				}
+				xp[u] = v;
+			}
+		}
+	};
+	auto store_stage = [&](int buf, uint64_t ibase) {
+#pragma unroll
+		for (int k = 0; k < LPT; k++) {
+			const int j = tid + 256 * k;
+			const int ib = j / (OT * 64), t = (j >> 6) % OT, rho = j & 31, pr = (j >> 5) & 1;
+			const uint64_t i0 = ibase + 4 * ib + 2 * pr;  // inputs of this pair: i0, i0 + 1
+			uint32_t wv = pre[k] & (i0 < i_end ? 0xFFFFu : 0u);
+			wv |= i0 + 1 < i_end ? (pre[k] & 0xFFFF0000u) : 0u;
+			if (FP4) {
+				sA[buf][ib][t][0][rho + 32 * pr] = g16_expand_fp4(wv);
+			} else {
+				sA[buf][ib][t][pr][rho] = g16_expand_i8(wv & 0xFFFFu);
+				sA[buf][ib][t][pr][rho + 32] = g16_expand_i8(wv >> 16);
+			}
+		}
+	};
+
+	if (i_begin < i_end) {  // uniform over the work-group
+		int buf = 0;
+		load_stage(i_begin);
+		store_stage(0, i_begin);
+		__syncthreads();
+		for (uint64_t ibase = i_begin; ibase < i_end; ibase += 4 * IB) {
+#pragma unroll
+			for (int u = 0; u < 4 * IB; u++) xc[u] = xp[u];
+			load_stage(ibase + 4 * IB);
+#pragma unroll
+			for (int ib = 0; ib < IB; ib++) {
+				// B operands of 4 inputs: fp4 lane h = inputs 2h, 2h+1; int8 step s,
+				// lane h = input 2s + h
+				typedef typename std::conditional<FP4, g16_v8i, g16_v4i>::type bop_t;
+				bop_t bf[KS][NB];
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++) {
+					const uint32_t e0 = (uint32_t)(xc[4 * ib + 0] >> (16 * nb)) & 0xFFFFu;
+					const uint32_t e1 = (uint32_t)(xc[4 * ib + 1] >> (16 * nb)) & 0xFFFFu;
+					const uint32_t e2 = (uint32_t)(xc[4 * ib + 2] >> (16 * nb)) & 0xFFFFu;
+					const uint32_t e3 = (uint32_t)(xc[4 * ib + 3] >> (16 * nb)) & 0xFFFFu;
+					if constexpr (FP4) {
+						const uint4 ex = g16_expand_fp4(h ? (e2 | (e3 << 16)) : (e0 | (e1 << 16)));
+						bf[0][nb][0] = (int)ex.x;
+						bf[0][nb][1] = (int)ex.y;
+						bf[0][nb][2] = (int)ex.z;
+						bf[0][nb][3] = (int)ex.w;
+#pragma unroll
+						for (int v = 4; v < 8; v++) bf[0][nb][v] = 0;
+					} else {
+						const uint4 x0 = g16_expand_i8(h ? e1 : e0);
+						const uint4 x1 = g16_expand_i8(h ? e3 : e2);
+						bf[0][nb][0] = (int)x0.x;
+						bf[0][nb][1] = (int)x0.y;
+						bf[0][nb][2] = (int)x0.z;
+						bf[0][nb][3] = (int)x0.w;
+						bf[KS - 1][nb][0] = (int)x1.x;
+						bf[KS - 1][nb][1] = (int)x1.y;
+						bf[KS - 1][nb][2] = (int)x1.z;
+						bf[KS - 1][nb][3] = (int)x1.w;
+					}
+				}
+#pragma unroll
+				for (int t = 0; t < OT; t++) {
+#pragma unroll
+					for (int s = 0; s < KS; s++) {
+						const uint4 av = sA[buf][ib][t][s][lane];
+						if constexpr (FP4) {
+							g16_v8i af;
+							af[0] = (int)av.x;
+							af[1] = (int)av.y;
+							af[2] = (int)av.z;
+							af[3] = (int)av.w;
+#pragma unroll
+							for (int v = 4; v < 8; v++) af[v] = 0;
+#pragma unroll
+							for (int nb = 0; nb < NB; nb++)
+								acc[t][nb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+								    af, bf[0][nb], acc[t][nb], 4, 4, 0, 128, 0, 128);
+						} else {
+							g16_v4i af;
+							af[0] = (int)av.x;
+							af[1] = (int)av.y;
+							af[2] = (int)av.z;
+							af[3] = (int)av.w;
+#pragma unroll
+							for (int nb = 0; nb < NB; nb++)
+								acc[t][nb] =
+								    __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[s][nb], acc[t][nb], 0, 0, 0);
+						}
+					}
+				}
+			}
+			store_stage(buf ^ 1, ibase + 4 * IB);
+			__syncthreads();
+			buf ^= 1;
+		}
+	}
+	// parity -> 16-bit words.  Lane (col, h) holds rows 8(j/4) + 4h + (j%4) of
+	// each 32-row tile: j < 8 are bits of output 2t, j >= 8 of output 2t + 1.
+	// After the exchange with lane col + 32, lane half h writes output 2t + h.
+#pragma unroll
+	for (int t = 0; t < OT; t++) {
+		xw_t ov = 0;
+#pragma unroll
+		for (int nb = 0; nb < NB; nb++) {
+			uint32_t b0 = 0, b1 = 0;
+#pragma unroll
+			for (int j = 0; j < 8; j++) {
+				const int o = 8 * (j >> 2) + 4 * h + (j & 3);
+				b0 |= ((uint32_t)acc[t][nb][j] & 1u) << o;
+				b1 |= ((uint32_t)acc[t][nb][j + 8] & 1u) << o;
+			}
+			b0 |= (uint32_t)__shfl_xor((int)b0, 32);
+			b1 |= (uint32_t)__shfl_xor((int)b1, 32);
+			ov |= (xw_t)(h ? b1 : b0) << (16 * nb);
+		}
+		const uint64_t e = r0 + 2 * t + h;
+		if (e >= a.R) continue;
+		uint16_t* dst;
+		bool xr;
+		if (a.partial) {
+			dst = a.partial + (g * a.R + e) * a.W;
+			xr = false;
+		} else {
+			dst = a.out_base + e * a.out_stride;
+			xr = a.accumulate != 0;
+		}
+		if (w0 + NB <= a.W && ((reinterpret_cast<uintptr_t>(dst + w0) & (2 * NB - 1)) == 0)) {
+			xw_t* q = reinterpret_cast<xw_t*>(dst + w0);
+			*q = xr ? (*q ^ ov) : ov;
+		} else {
+#pragma unroll
+			for (int nb = 0; nb < NB; nb++) {
+				const uint64_t w = w0 + nb;
+				const uint16_t v = (uint16_t)(ov >> (16 * nb));
+				if (w < a.W) dst[w] = xr ? (uint16_t)(dst[w] ^ v) : v;
+			}
+		}
+	}
+}
+
+hipError_t launch_gf16_rows(const uint16_t* coeff, uint64_t ld, uint64_t N, uint64_t R, uint32_t* rows,
+                            hipStream_t s) {
+	const uint64_t total = R * ((N + 1) / 2) * 16;
+	if (!total) return hipSuccess;
+	uint64_t blocks = (total + 255) / 256;
+	if (blocks > 65536) blocks = 65536;
+	hipLaunchKernelGGL(k_gf16_rows, dim3((unsigned)blocks), dim3(256), 0, s, coeff, ld, N, R, rows);
+	return hipGetLastError();
+}
+
+template <int OT, int NB, bool FP4>
+static hipError_t launch_g16m_t(const Gf16MfmaArgs& a, uint64_t groups, hipStream_t s) {
+	constexpr int IB = 4;  // 16 inputs per stage
+	const uint64_t span = 4ull * 32 * NB;
+	const uint64_t nx = (a.W + span - 1) / span, Z = (a.R + 2 * OT - 1) / (2 * OT);
+	const uint64_t blocks = (nx + 7) / 8 * 8 * Z;
+	if (blocks > 0x7fffffffu || groups > 65535) return hipErrorInvalidValue;
+	hipLaunchKernelGGL((k_gf16_mfma<OT, NB, IB, FP4>), dim3((unsigned)blocks, (unsigned)groups, 1),
+	                   dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+// (OT output tiles, NB column blocks, fp4): OT * NB * 16 accumulators per lane
+#define PARHIP_G16M_VARIANTS(X) \
+	X(4, 4, true)               \
+	X(4, 2, true)               \
+	X(8, 2, true)               \
+	X(4, 4, false)              \
+	X(4, 2, false)              \
+	X(8, 2, false)
+
+bool gf16_mfma_supported(int ot, int nb, bool fp4) {
+#define X(O_, N_, F_) \
+	if (ot == O_ && nb == N_ && fp4 == F_) return true;
+	PARHIP_G16M_VARIANTS(X)
+#undef X
+	return false;
+}
+
+hipError_t launch_gf16_mfma(const Gf16MfmaArgs& a, uint64_t groups, int ot, int nb, bool fp4,
+                            hipStream_t s) {
+#define X(O_, N_, F_) \
+	if (ot == O_ && nb == N_ && fp4 == F_) return launch_g16m_t<O_, N_, F_>(a, groups, s);
+	PARHIP_G16M_VARIANTS(X)
+#undef X
+	return hipErrorInvalidValue;
+}
+
+}  // namespace parhip

@@ -83,6 +83,24 @@ struct Gf16Args {
 	int accumulate;
 };
 
+// GF(2^16) contraction as a GF(2) bit-matrix on MFMA (gf16_mfma.hip).
+struct Gf16MfmaArgs {
+	const uint16_t* in_base;
+	uint64_t in_stride;   // words
+	uint16_t* out_base;
+	uint64_t out_stride;  // words
+	const uint32_t* rows; // k_gf16_rows: (R x ceil(N/2) x 16) row words
+	uint64_t N, R, W;
+	uint64_t in_per_group;  // multiple of 4
+	uint16_t* partial;      // if non-null: G x R x W partials
+	int accumulate;
+};
+hipError_t launch_gf16_rows(const uint16_t* coeff, uint64_t ld, uint64_t N, uint64_t R, uint32_t* rows,
+                            hipStream_t s);
+bool gf16_mfma_supported(int ot, int nb, bool fp4);
+hipError_t launch_gf16_mfma(const Gf16MfmaArgs& a, uint64_t groups, int ot, int nb, bool fp4,
+                            hipStream_t s);
+
 hipError_t launch_gf16_coeffs(uint16_t* coeff, const uint16_t* input_nums, uint64_t N,
                               const uint16_t* exps, uint64_t R, const uint16_t* exp_tab,
                               const uint16_t* inlog_tab, hipStream_t s);
