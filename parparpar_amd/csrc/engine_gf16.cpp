@@ -91,10 +91,10 @@ static Plan16 plan16(uint64_t N, uint64_t R, uint64_t W) {
 
 // GF(2) bit-matrix strategy on MFMA (gf16_mfma.hip): row words of every
 // coefficient, then the MFMA contraction; split-K partials folded as for the
-// tables.  Output tiles of 8 outputs and 4 column blocks of 32 words per wave
-// (measured best for both types at 32768 x 32 KiB, R = 512).
+// tables.  Output tiles of 8 outputs and 2 column blocks of 32 words per wave,
+// two waves per SIMD (measured best for both types at 32768 x 32 KiB, R = 512).
 static int run_gf16_mfma(const Gf16Args& m, bool fp4, hipStream_t s) {
-	int ot = 4, nb = 4, trt, twpt, tib;
+	int ot = 4, nb = 2, trt, twpt, tib;
 	tune_shape_override(&trt, &twpt, &tib);
 	if ((trt || twpt) && gf16_mfma_supported(trt ? trt : ot, twpt ? twpt : nb, fp4)) {
 		ot = trt ? trt : ot;

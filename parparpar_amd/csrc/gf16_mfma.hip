@@ -83,8 +83,8 @@ __global__ void k_gf16_rows(const uint16_t* __restrict__ coeff, uint64_t ld, uin
 // use clamped, in-bounds addresses: words past W and outputs past R only reach
 // accumulator columns / rows that are never stored, and inputs past the range
 // get zero rows when they are stored to LDS.
-template <int OT, int NB, int IB, bool FP4>
-__global__ __launch_bounds__(256, 1) void k_gf16_mfma(Gf16MfmaArgs a) {
+template <int OT, int NB, int IB, bool FP4, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_gf16_mfma(Gf16MfmaArgs a) {
 	constexpr int KS = FP4 ? 1 : 2;             // MFMA K steps per 4 inputs
 	constexpr int LPT = (IB * OT * 64 + 255) / 256;  // row words loaded per thread per stage
 	static_assert((IB * OT * 64) % 256 == 0, "whole waves stage row words");
@@ -309,41 +309,30 @@ hipError_t launch_gf16_rows(const uint16_t* coeff, uint64_t ld, uint64_t N, uint
 	return hipGetLastError();
 }
 
-template <int OT, int NB, bool FP4>
+template <int OT, int NB, int IB, bool FP4, int OCC>
 static hipError_t launch_g16m_t(const Gf16MfmaArgs& a, uint64_t groups, hipStream_t s) {
-	constexpr int IB = 4;  // 16 inputs per stage
 	const uint64_t span = 4ull * 32 * NB;
 	const uint64_t nx = (a.W + span - 1) / span, Z = (a.R + 2 * OT - 1) / (2 * OT);
 	const uint64_t blocks = (nx + 7) / 8 * 8 * Z;
 	if (blocks > 0x7fffffffu || groups > 65535) return hipErrorInvalidValue;
-	hipLaunchKernelGGL((k_gf16_mfma<OT, NB, IB, FP4>), dim3((unsigned)blocks, (unsigned)groups, 1),
+	hipLaunchKernelGGL((k_gf16_mfma<OT, NB, IB, FP4, OCC>), dim3((unsigned)blocks, (unsigned)groups, 1),
 	                   dim3(256), 0, s, a);
 	return hipGetLastError();
 }
 
-// (OT output tiles, NB column blocks, fp4): OT * NB * 16 accumulators per lane
-#define PARHIP_G16M_VARIANTS(X) \
-	X(4, 4, true)               \
-	X(4, 2, true)               \
-	X(8, 2, true)               \
-	X(4, 4, false)              \
-	X(4, 2, false)              \
-	X(8, 2, false)
-
-bool gf16_mfma_supported(int ot, int nb, bool fp4) {
-#define X(O_, N_, F_) \
-	if (ot == O_ && nb == N_ && fp4 == F_) return true;
-	PARHIP_G16M_VARIANTS(X)
-#undef X
-	return false;
+// The (OT, NB) pair selects the variant; (4, 2) runs the two-waves-per-SIMD form.
+bool gf16_mfma_supported(int ot, int nb, bool /*fp4*/) {
+	return (ot == 4 && (nb == 4 || nb == 2)) || (ot == 8 && nb == 2);
 }
 
 hipError_t launch_gf16_mfma(const Gf16MfmaArgs& a, uint64_t groups, int ot, int nb, bool fp4,
                             hipStream_t s) {
-#define X(O_, N_, F_) \
-	if (ot == O_ && nb == N_ && fp4 == F_) return launch_g16m_t<O_, N_, F_>(a, groups, s);
-	PARHIP_G16M_VARIANTS(X)
-#undef X
+	if (ot == 4 && nb == 4) return fp4 ? launch_g16m_t<4, 4, 4, true, 1>(a, groups, s)
+	                                   : launch_g16m_t<4, 4, 4, false, 1>(a, groups, s);
+	if (ot == 4 && nb == 2) return fp4 ? launch_g16m_t<4, 2, 2, true, 2>(a, groups, s)
+	                                   : launch_g16m_t<4, 2, 2, false, 2>(a, groups, s);
+	if (ot == 8 && nb == 2) return fp4 ? launch_g16m_t<8, 2, 4, true, 1>(a, groups, s)
+	                                   : launch_g16m_t<8, 2, 4, false, 1>(a, groups, s);
 	return hipErrorInvalidValue;
 }
 
