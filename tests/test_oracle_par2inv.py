@@ -11,7 +11,12 @@ vendored.  The restatement is pinned instead by
   e + 17 make the leading 2 x 2 block singular; the second pivot is zero and
   that exponent is dropped (gfmat_inv.cpp:638-646), or Compute fails when no
   spare exponent is left;
-- sorting of the exponents and their resize to numRec.
+- sorting of the exponents and their resize to numRec;
+- the fixed cases of the reference's own inversion test
+  (test/gf16/test-inv.cpp:87-147) and random cases of the same shape, whose
+  success is the field fact par2cmdline computes there (every leading pivot
+  nonzero) and whose factors must satisfy the repair identity, both computed
+  by tests/gf16_math.py from the PAR2 specification without the oracle.
 """
 import numpy as np
 import pytest
@@ -66,4 +71,81 @@ def test_not_enough_recovery(oracle):
     valid = np.ones(10, dtype=bool)
     valid[[1, 2, 3]] = False
     ok, used, _ = oracle.gf16_rec_matrix(valid, [0, 1])
+    assert not ok
+
+
+from par2inv_cases import (FLAW, REFERENCE_FIXED, REFERENCE_FIXED_OK, expected_ok,
+                           random_cases)
+
+
+@pytest.mark.parametrize("case", range(len(REFERENCE_FIXED)))
+def test_reference_fixed_cases(oracle, case):
+    import gf16_math as G
+    valid, exps = REFERENCE_FIXED[case]
+    want = expected_ok(valid, exps)
+    assert want == REFERENCE_FIXED_OK[case]
+    ok, used, F = oracle.gf16_rec_matrix(valid, exps)
+    assert ok == want
+    if ok:
+        assert list(used) == sorted(exps)
+        assert G.check_factors(valid, used, F)
+
+
+def test_reference_flaw_is_singular():
+    """Inputs 0 and 6554 have constants 2^1 and 2^13108: their ratio 2^13107 has
+    order 65535 / 13107 = 5, so with both missing, exponents 0 and 5 give two
+    proportional rows, a true rank deficiency, while 0 and 6 do not."""
+    import gf16_math as G
+    assert G.INPUT_LOGS[6554] - G.INPUT_LOGS[0] == 65535 // 5
+    M = G.coeff_matrix([0, 6554], [0, 5])
+    assert G.rank(M) == 1
+    assert G.rank(G.coeff_matrix([0, 6554], [0, 6])) == 2
+
+
+def test_reference_flaw_drops_an_exponent(oracle):
+    """test/gf16/test-inv.cpp:103-129: with exponents 0, 5, 6 the flawed pair
+    still inverts after dropping 0 or 5, and 6 is kept."""
+    import gf16_math as G
+    ok, used, F = oracle.gf16_rec_matrix(FLAW, [0, 5, 6])
+    assert ok and len(used) == 2
+    assert used[0] in (0, 5) and used[1] == 6
+    assert G.check_factors(FLAW, used, F)
+
+
+def test_reference_random_small(oracle):
+    import gf16_math as G
+    cases = random_cases(0x01020304, [2, 100], lambda n: [0.1, 0.5, 0.9], lambda n: 10)
+    assert len(cases) > 40
+    for valid, exps in cases:
+        ok, used, F = oracle.gf16_rec_matrix(valid, exps)
+        assert ok == expected_ok(valid, exps)
+        if ok:
+            rows = None if F.shape[0] <= 16 else [0, F.shape[0] // 2, F.shape[0] - 1]
+            assert G.check_factors(valid, used, F, rows)
+
+
+def test_factor_check_catches_a_wrong_factor(oracle):
+    """A single wrong factor, in a valid-input column or a recovery column, or
+    two rows swapped, fails the repair identity."""
+    import gf16_math as G
+    valid = np.ones(30, dtype=bool)
+    valid[[2, 9, 17]] = False
+    ok, used, F = oracle.gf16_rec_matrix(valid, [4, 11, 40])
+    assert ok and G.check_factors(valid, used, F)
+    for r, c in [(0, 0), (1, 26), (2, 29)]:
+        bad = F.copy()
+        bad[r, c] ^= 0x100
+        assert not G.check_factors(valid, used, bad)
+    assert not G.check_factors(valid, used, F[[1, 0, 2]])
+    assert not G.check_factors(valid, used[::-1], F)
+
+
+def test_reference_random_singular_at_scale(oracle):
+    """A random case of 1234 inputs with 1121 missing whose exponents leave a
+    zero leading pivot: the field check and the oracle both refuse it."""
+    cases = random_cases(1234 * 7 + 3, [1234], lambda n: [0.1, 0.5, 0.9], lambda n: 2)
+    valid, exps = cases[5]
+    assert int((~valid).sum()) == 1121
+    assert not expected_ok(valid, exps)
+    ok, used, _ = oracle.gf16_rec_matrix(valid, exps)
     assert not ok

@@ -85,3 +85,51 @@ def test_arguments(gpu):
         par2.rec_matrix_compute(np.ones(4, dtype=bool), [1, 2])       # nothing missing
     ok, used, _ = par2.rec_matrix_compute(np.array([1, 0, 0], dtype=bool), [7])
     assert not ok
+
+
+@pytest.mark.parametrize("case", range(13))
+def test_reference_fixed_cases(gpu, oracle, case):
+    """The fixed cases of test/gf16/test-inv.cpp:87-147 on the device: same
+    success, exponents and factors as the oracle."""
+    from parparpar_amd import par2
+    from par2inv_cases import REFERENCE_FIXED, REFERENCE_FIXED_OK
+    valid, exps = REFERENCE_FIXED[case]
+    ok, used, F = par2.rec_matrix_compute(valid, exps)
+    ok_o, used_o, F_o = oracle.gf16_rec_matrix(valid, exps)
+    assert ok == ok_o == REFERENCE_FIXED_OK[case]
+    assert np.array_equal(used, used_o)
+    if ok:
+        assert np.array_equal(F, F_o)
+
+
+def test_reference_flaw_drops_an_exponent(gpu, oracle):
+    """test/gf16/test-inv.cpp:103-129: exponents 0, 5, 6 for the flawed pair."""
+    from parparpar_amd import par2
+    from par2inv_cases import FLAW
+    ok, used, F = par2.rec_matrix_compute(FLAW, [6, 0, 5])
+    assert ok and len(used) == 2 and used[0] in (0, 5) and used[1] == 6
+    ok_o, used_o, F_o = oracle.gf16_rec_matrix(FLAW, [6, 0, 5])
+    assert ok_o and np.array_equal(used, used_o) and np.array_equal(F, F_o)
+
+
+@pytest.mark.parametrize("sizes,probs,rounds", [
+    ([2, 100], [0.1, 0.5, 0.9], 10),
+    ([1234], [0.1, 0.5, 0.9], 2),
+    ([32768], [0.01], 1),
+])
+def test_reference_random(gpu, oracle, sizes, probs, rounds):
+    """Random cases shaped like test/gf16/test-inv.cpp:151-196 (input counts 2,
+    100, 1234 and 32768, one exponent per missing input): device and oracle
+    agree bit for bit."""
+    from parparpar_amd import par2
+    from par2inv_cases import random_cases
+    cases = random_cases(sizes[0] * 7 + len(probs), sizes, lambda n: probs, lambda n: rounds)
+    assert cases
+    for valid, exps in cases:
+        print(f"n={valid.size} missing={int((~valid).sum())}", flush=True)
+        ok, used, F = par2.rec_matrix_compute(valid, exps)
+        ok_o, used_o, F_o = oracle.gf16_rec_matrix(valid, exps)
+        assert ok == ok_o
+        assert np.array_equal(used, used_o)
+        if ok:
+            assert np.array_equal(F, F_o)
