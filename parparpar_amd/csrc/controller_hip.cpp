@@ -13,6 +13,41 @@ namespace parhip {
 
 int run_gf16_contraction(Gf16Args a, hipStream_t s);
 
+// Checksum word of a staged slice: sum over words w of x^(n'-1-w) * word_w in
+// GF(2^16) (n' = word count rounded up to a multiple of 4), the positional
+// checksum of the reference's staging (gf16/gf16_checksum_generic.h: Horner
+// steps of a multiply by x and an XOR per block).  Linear over GF(2^16), so the
+// muladd carries it: cksum(sum_i c_i * in_i) = sum_i c_i * cksum(in_i).  Unlike
+// a plain XOR it also catches words that land in the wrong place.  Computed as
+// four interleaved 16-bit lanes of a 64-bit word, each stepped by x^4 per
+// 4-word chunk, then folded as x^3 L0 + x^2 L1 + x L2 + L3.
+static inline uint64_t lanes_mulx4(uint64_t v) {
+	const uint64_t t = (v >> 12) & 0x000F000F000F000Full;  // x^16 = x^12 + x^3 + x + 1
+	return ((v << 4) & 0xFFF0FFF0FFF0FFF0ull) ^ (t << 12) ^ (t << 3) ^ (t << 1) ^ t;
+}
+
+static inline uint16_t mulx16(uint16_t a) {
+	return (uint16_t)((a << 1) ^ ((a & 0x8000u) ? 0x100Bu : 0u));
+}
+
+static uint16_t slice_checksum(const uint16_t* p, size_t n) {
+	uint64_t lanes = 0;
+	size_t w = 0;
+	for (; w + 4 <= n; w += 4) {
+		uint64_t v;
+		std::memcpy(&v, p + w, 8);
+		lanes = lanes_mulx4(lanes) ^ v;
+	}
+	if (w < n) {
+		uint64_t v = 0;
+		std::memcpy(&v, p + w, (n - w) * 2);
+		lanes = lanes_mulx4(lanes) ^ v;
+	}
+	uint16_t ck = 0;
+	for (int l = 0; l < 4; l++) ck = (uint16_t)(mulx16(ck) ^ (uint16_t)(lanes >> (16 * l)));
+	return ck;
+}
+
 static std::future<void> ready_future() {
 	std::promise<void> p;
 	p.set_value();
@@ -160,18 +195,7 @@ std::future<void> PAR2ProcHIP::addInputT(const void* buffer, size_t size, T numO
 	if (size > payloadWords * 2) size = payloadWords * 2;
 	std::memcpy(slot, buffer, size);
 	std::memset(reinterpret_cast<char*>(slot) + size, 0, (words * 2) - size);
-	// linear checksum word: XOR of all payload words (GF(2^16)-linear, so the
-	// muladd carries it along: cksum(out) == XOR_i c_i * cksum(in_i))
-	uint64_t x = 0;
-	size_t w = 0;
-	for (; w + 4 <= payloadWords; w += 4) {
-		uint64_t v;
-		std::memcpy(&v, slot + w, 8);
-		x ^= v;
-	}
-	uint16_t ck = (uint16_t)(x ^ (x >> 16) ^ (x >> 32) ^ (x >> 48));
-	for (; w < payloadWords; w++) ck ^= slot[w];
-	slot[payloadWords] = ck;
+	slot[payloadWords] = slice_checksum(slot, payloadWords);
 	setCoeffs(st, currentStagingInputs, numOrCoeffs);
 	currentStagingInputs++;
 	if (currentStagingInputs >= inputBatchSize || flushNow) flush();
@@ -276,10 +300,8 @@ std::future<bool> PAR2ProcHIP::getOutput(unsigned index, void* output) {
 		pr.set_value(false);
 		return pr.get_future();
 	}
-	uint16_t ck = 0;
-	for (size_t w = 0; w + 1 < words; w++) ck ^= tmp[w];
 	std::memcpy(output, tmp.data(), currentSliceSize);
-	pr.set_value(ck == tmp[words - 1]);
+	pr.set_value(slice_checksum(tmp.data(), words - 1) == tmp[words - 1]);
 	return pr.get_future();
 }
 

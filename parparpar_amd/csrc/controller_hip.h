@@ -5,10 +5,10 @@
 // the same member names, argument meaning and add/flush/end/get sequencing, so
 // that PAR2Proc (gf16/controller.cpp) can drive it beside PAR2ProcCPU and
 // PAR2ProcOCL.  Inside, inputs are staged in pinned host memory with a linear
-// checksum word appended, copied to HBM a batch at a time, and multiplied into
-// the device-resident recovery slices by the GF(2^16) kernels; the first batch
-// of a pass overwrites the outputs and later batches XOR into them
-// (controller_ocl.cpp:626-634).  getOutput() copies one recovery slice back and
+// positional checksum word appended, copied to HBM a batch at a time, and
+// multiplied into the device-resident recovery slices by the GF(2^16) kernels;
+// the first batch of a pass overwrites the outputs and later batches XOR into
+// them (controller_ocl.cpp:626-634).  getOutput() copies one recovery slice back and
 // reports whether its checksum word still matches (the reference's cksumValid).
 #pragma once
 #include <hip/hip_runtime.h>
