@@ -414,12 +414,17 @@ def main_batch(args):
         "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded uniform random bytes, resident in HBM)",
         "config": {"workload": "batched uneven PAR3 create: %d jobs, %.2f GiB total, size LogUniform[4 MiB, 512 MiB], "
                                "slice {4,8,32,1024} KiB, R U{4..64}" % (len(jobs), gib),
-                   "method": "persistent k_batch_persistent + device tile queue + hipGraph, autotuned variants"},
+                   "method": "persistent k_batch_persistent + device tile queue + hipGraph; autotuned per shape: "
+                             "table launch variants, or the GF(2) bit-matrix on MFMA where eligible"},
         "plain_GiBps": round(gib / res["plain"], 3),
         "batched_GiBps": round(gib / res["batched"], 3),
         "batched_vs_plain": round(res["plain"] / res["batched"], 3),
         "plan_s": round(plan_s, 3),
         "batch_info": b.info(),
+        # strategy per job after autotuning (MFMA timed where its rows fit: 64 R <= words)
+        "jobs_by_method": {"table": sum(b.job_method(k) == P.METHOD_TABLE for k in range(len(jobs))),
+                           "mfma": sum(b.job_method(k) == P.METHOD_MFMA for k in range(len(jobs))),
+                           "mfma_candidates": sum(64 * y.shape[0] <= x.shape[1] for (x, y, _, _) in jobs)},
         "bit_exact_batched_vs_plain": bool(exact),
         "cpu_baseline": None,
     }

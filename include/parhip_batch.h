@@ -33,7 +33,15 @@ typedef struct {
 typedef struct parhip_batch parhip_batch;
 
 #define PARHIP_BATCH_GRAPH 1u    /* capture the launch sequence in a hipGraph */
-#define PARHIP_BATCH_AUTOTUNE 2u /* time the launch variants per job shape (cached) */
+/* Time the candidates per job shape (cached): the table launch variants in the
+ * persistent kernel, and the strategy itself where the GF(2) bit-matrix on MFMA
+ * qualifies (its rows, 512 B per coefficient, no larger than the job's input:
+ * 64 * numRecovery <= blockSize64): the job alone as full-device launches of
+ * the table kernel and of the MFMA kernel.  A job that goes to MFMA runs as its
+ * own launches after the persistent kernel. */
+#define PARHIP_BATCH_AUTOTUNE 2u
+/* Run every job that qualifies as above on the MFMA strategy, untimed. */
+#define PARHIP_BATCH_MFMA 4u
 
 /* Plan a batch: per-job launch variant (autotuned when asked), the device tile
  * queue, coefficient storage; optionally capture the graph. Buffers named by
@@ -45,13 +53,20 @@ int parhip_batch_info(const parhip_batch *b, uint64_t *ntiles, uint64_t *natomic
                       uint64_t *coeff_bytes);
 /* Launch variant chosen for job `job` (its index in the array given to
  * parhip_batch_create): outputs per work-group, words per thread, inputs per
- * LDS batch.  PARHIP_ERR_ARG for an empty job (no inputs, outputs or words). */
+ * LDS batch (MFMA: outputs per work-group, 32-word column blocks per wave,
+ * inputs per stage).  PARHIP_ERR_ARG for an empty job (no inputs, outputs or
+ * words). */
 int parhip_batch_job_config(const parhip_batch *b, size_t job, int *rt, int *wpt, int *ib);
+/* Strategy of job `job`: PARHIP_METHOD_TABLE or PARHIP_METHOD_MFMA (include/parhip.h);
+ * PARHIP_ERR_ARG for an empty job. */
+int parhip_batch_job_method(const parhip_batch *b, size_t job);
 void parhip_batch_destroy(parhip_batch *b);
 
-/* Autotuner cache: look up a tuned shape, persist, reload, clear. */
+/* Autotuner cache: look up a tuned shape (launch variant, or its strategy),
+ * persist, reload, clear. */
 int parhip_autotune_lookup(size_t numInputs, size_t numRecovery, size_t blockSize64, int *rt,
                            int *wpt, int *ib, double *ns_per_mac);
+int parhip_autotune_lookup_method(size_t numInputs, size_t numRecovery, size_t blockSize64);
 int parhip_autotune_save(const char *path);
 int parhip_autotune_load(const char *path);
 int parhip_autotune_clear(void);

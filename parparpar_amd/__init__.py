@@ -90,8 +90,10 @@ _SIGS = [
     ("parhip_batch_run", _int, [_vp, _vp]),
     ("parhip_batch_info", _int, [_vp, _vp, _vp, _vp]),
     ("parhip_batch_job_config", _int, [_vp, _sz, _vp, _vp, _vp]),
+    ("parhip_batch_job_method", _int, [_vp, _sz]),
     ("parhip_batch_destroy", None, [_vp]),
     ("parhip_autotune_lookup", _int, [_sz, _sz, _sz, _vp, _vp, _vp, _vp]),
+    ("parhip_autotune_lookup_method", _int, [_sz, _sz, _sz]),
     ("parhip_autotune_save", _int, [ctypes.c_char_p]),
     ("parhip_autotune_load", _int, [ctypes.c_char_p]),
     ("parhip_autotune_clear", _int, []),

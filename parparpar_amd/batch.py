@@ -7,6 +7,7 @@ from . import ParhipError, check, lib
 
 GRAPH = 1
 AUTOTUNE = 2
+MFMA = 4
 
 
 class _Job(ctypes.Structure):
@@ -21,7 +22,7 @@ class Batch:
     """jobs: list of (inputs (N, W) int64 tensor, recovery (R, W) int64 tensor,
     firstInput, firstRecovery)."""
 
-    def __init__(self, jobs, graph=True, autotune=True):
+    def __init__(self, jobs, graph=True, autotune=True, mfma=False):
         arr = (_Job * len(jobs))()
         for k, (inp, rec, fi, fr) in enumerate(jobs):
             n, w = inp.shape
@@ -30,7 +31,7 @@ class Batch:
                 raise ValueError("job %d: bad tensor layout" % k)
             arr[k] = _Job(inp.data_ptr(), n, inp.stride(0), rec.data_ptr(), r, rec.stride(0), w, fi, fr)
         self._keep = jobs
-        flags = (GRAPH if graph else 0) | (AUTOTUNE if autotune else 0)
+        flags = (GRAPH if graph else 0) | (AUTOTUNE if autotune else 0) | (MFMA if mfma else 0)
         self._h = lib().parhip_batch_create(ctypes.addressof(arr), len(jobs), flags)
         if not self._h:
             raise ParhipError(-4, lib().parhip_last_error().decode())
@@ -48,6 +49,13 @@ class Batch:
         rt, wpt, ib = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().parhip_batch_job_config(self._h, k, ctypes.byref(rt), ctypes.byref(wpt), ctypes.byref(ib)))
         return rt.value, wpt.value, ib.value
+
+    def job_method(self, k):
+        """METHOD_TABLE or METHOD_MFMA: the strategy job k runs on."""
+        rc = lib().parhip_batch_job_method(self._h, k)
+        if rc < 0:
+            check(rc)
+        return rc
 
     def close(self):
         if self._h:

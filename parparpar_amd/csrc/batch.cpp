@@ -9,7 +9,10 @@
 // zeroing of split jobs' outputs, queue reset, persistent kernel -- is
 // captured in a hipGraph and replayed.  Jobs whose input range is split over
 // several tiles accumulate with 64-bit atomic XOR (order-independent: the
-// bytes are identical to the unsplit computation).
+// bytes are identical to the unsplit computation).  A job can instead run on
+// the GF(2) bit-matrix (MFMA) strategy, as its own launch sequence after the
+// persistent kernel, when the autotuner measures it faster for the job's
+// shape or the caller asks for it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,8 +34,31 @@ namespace parhip {
 struct CfgInfo {
 	int rt, wpt, ib;
 };
+// table variants: outputs per work-group, words per thread, inputs per LDS
+// batch; MFMA: outputs per work-group, 32-word column blocks per wave, inputs
+// per stage of the fp4 kernel's default shape
 static const CfgInfo kCfg[BATCH_CFG_COUNT] = {
-	{8, 4, 4}, {8, 2, 4}, {0, 0, 0} /* unused */, {16, 2, 2}, {4, 4, 4}};
+	{8, 4, 4}, {8, 2, 4}, {4, 2, 4} /* MFMA */, {16, 2, 2}, {4, 4, 4}};
+
+// The MFMA strategy needs the 64 bit-matrix rows of every coefficient (512 B
+// each) beside the job; it is offered only where they are no larger than the
+// job's input (64 R <= words per slice).
+static bool mfma_eligible(const BatchJob& jb) { return 64 * jb.R <= jb.W; }
+
+static MacArgs mfma_args(const BatchJob& jb, uint64_t* out, uint64_t out_stride) {
+	MacArgs m{};
+	m.in_base = jb.in;
+	m.in_stride = jb.in_stride;
+	m.out_base = out;
+	m.out_stride = out_stride;
+	m.coeff = jb.coeff;
+	m.coeff_ld = jb.N;
+	m.N = jb.N;
+	m.R = jb.R;
+	m.W = jb.W;
+	m.accumulate = 0;
+	return m;
+}
 
 static std::vector<int> candidates(uint64_t R, uint64_t W) {
 	std::vector<int> c;
@@ -149,7 +175,8 @@ static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_
 		}
 	}
 	std::vector<int> cand = candidates(jb.R, jb.W);
-	if (cand.size() == 1) {
+	const bool try_mfma = mfma_eligible(jb);
+	if (cand.size() == 1 && !try_mfma) {
 		*cfg_out = cand[0];
 		std::lock_guard<std::mutex> lk(g_tune_mu);
 		g_tune[key] = {cand[0], 0.0};
@@ -186,6 +213,45 @@ static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_
 			best = c;
 		}
 	}
+	MfmaPlan mp;
+	if (try_mfma && plan_mfma(jb.N, jb.R, jb.W, true, false, &mp) == PARHIP_OK) {
+		// Strategy choice on equal terms: the job alone as full-device launches
+		// of each strategy (table kernel with its split-K plan, and rows + MFMA
+		// kernel + fold), so neither is judged by how well one small job fills
+		// a persistent grid.
+		DevBuf mws;
+		if ((rc = mws.alloc(mp.rows_bytes + mp.part_bytes, s))) return rc;
+		const MacArgs m = mfma_args(jb, dout.as<uint64_t>(), jb.W);
+		float ms_tab = 1e30f, ms_mfma = 1e30f;
+		hipEvent_t a, b;
+		hipEventCreate(&a);
+		hipEventCreate(&b);
+		const bool timing = prof_set(false);  // keep the caller's kernel timings clean
+		hipError_t e = hipSuccess;
+		for (int which = 0; which < 2 && !rc && e == hipSuccess; which++)
+			for (int rep = 0; rep < 3 && !rc && e == hipSuccess; rep++) {
+				hipEventRecord(a, s);
+				if (which == 0)
+					rc = run_contraction_tables(m, s);
+				else
+					e = enqueue_mfma(m, mp, mws.p, s);
+				hipEventRecord(b, s);
+				if (!rc && e == hipSuccess) e = hipEventSynchronize(b);
+				float ms = 0;
+				if (!rc && e == hipSuccess) hipEventElapsedTime(&ms, a, b);
+				float& best_ms = which ? ms_mfma : ms_tab;
+				if (rep && ms < best_ms) best_ms = ms;  // first run warms up
+			}
+		prof_set(timing);
+		hipEventDestroy(a);
+		hipEventDestroy(b);
+		if (rc) return rc;
+		if (e != hipSuccess) return hip_fail(e, "autotune mfma launch");
+		if (ms_mfma < ms_tab) {
+			best_ns = ms_mfma * 1e6 / macs;
+			best = BATCH_CFG_MFMA;
+		}
+	}
 	hipStreamSynchronize(s);
 	std::lock_guard<std::mutex> lk(g_tune_mu);
 	g_tune[key] = {best, best_ns};
@@ -216,7 +282,26 @@ struct parhip_batch {
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	uint64_t tile_counts[BATCH_CFG_COUNT] = {0};
+	struct MfmaJob {
+		size_t job;        // index in jobs
+		MfmaPlan plan;
+		void* ws;          // bit-matrix rows + split-K partials, owned
+	};
+	std::vector<MfmaJob> mfma;
 };
+
+// The batch's launch sequence: every job's coefficients, zeroing of split
+// jobs, queue reset and persistent kernel, then the MFMA jobs' launches.
+static hipError_t batch_enqueue(const parhip_batch* b, hipStream_t s) {
+	hipError_t e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), b->dtiles, b->ntiles, b->dcounter,
+	                            b->dzero, b->nzero, b->coeff_total, b->blocks, s);
+	for (size_t k = 0; k < b->mfma.size() && e == hipSuccess; k++) {
+		const auto& mj = b->mfma[k];
+		const BatchJob& jb = b->jobs[mj.job];
+		e = enqueue_mfma(mfma_args(jb, jb.out, jb.out_stride), mj.plan, mj.ws, s);
+	}
+	return e;
+}
 
 static void batch_free(parhip_batch* b) {
 	if (!b) return;
@@ -229,6 +314,8 @@ static void batch_free(parhip_batch* b) {
 	if (b->dcounter) hipFree(b->dcounter);
 	if (b->dzero) hipFree(b->dzero);
 	if (b->dcoeff) hipFree(b->dcoeff);
+	for (auto& mj : b->mfma)
+		if (mj.ws) hipFree(mj.ws);
 	if (b->cap) hipStreamDestroy(b->cap);
 	delete b;
 }
@@ -304,11 +391,27 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 	for (size_t j = 0; j < b->jobs.size(); j++) {
 		const BatchJob& jb = b->jobs[j];
 		int cfg = jb.R <= 4 ? BATCH_CFG_4_4_4 : (jb.W <= 512 ? BATCH_CFG_8_2_4 : BATCH_CFG_8_4_4);
-		if (flags & PARHIP_BATCH_AUTOTUNE) {
+		if ((flags & PARHIP_BATCH_MFMA) && mfma_eligible(jb)) {
+			cfg = BATCH_CFG_MFMA;
+		} else if (flags & PARHIP_BATCH_AUTOTUNE) {
 			if (autotune_job(jb, b->blocks, b->cap, &cfg)) {
 				batch_free(b);
 				return nullptr;
 			}
+		}
+		if (cfg == BATCH_CFG_MFMA) {
+			parhip_batch::MfmaJob mj{j, MfmaPlan{}, nullptr};
+			if (plan_mfma(jb.N, jb.R, jb.W, true, false, &mj.plan) != PARHIP_OK) {
+				batch_free(b);
+				return nullptr;
+			}
+			e = hipMalloc(&mj.ws, mj.plan.rows_bytes + mj.plan.part_bytes);
+			if (e != hipSuccess) {
+				hip_fail(e, "batch mfma workspace");
+				batch_free(b);
+				return nullptr;
+			}
+			b->mfma.push_back(mj);
 		}
 		b->cfg[j] = cfg;
 	}
@@ -316,6 +419,7 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 	std::vector<std::vector<BatchTile>> per(b->jobs.size());
 	std::vector<uint32_t> zero_ids;
 	for (size_t j = 0; j < b->jobs.size(); j++) {
+		if (b->cfg[j] == BATCH_CFG_MFMA) continue;  // own launches, no tiles
 		bool atomic = false;
 		plan_job((uint32_t)j, b->jobs[j], b->cfg[j], per[j], &atomic);
 		b->tile_counts[b->cfg[j]] += per[j].size();
@@ -336,14 +440,15 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 			if (pos[j] < per[j].size()) tiles.push_back(per[j][pos[j]++]);
 	b->ntiles = (uint32_t)tiles.size();
 	b->nzero = (uint32_t)zero_ids.size();
-	e = hipMalloc((void**)&b->dtiles, tiles.size() * sizeof(BatchTile));
+	if (!tiles.empty()) e = hipMalloc((void**)&b->dtiles, tiles.size() * sizeof(BatchTile));
 	if (e == hipSuccess && b->nzero) e = hipMalloc((void**)&b->dzero, b->nzero * sizeof(uint32_t));
 	if (e != hipSuccess) {
 		hip_fail(e, "batch allocation");
 		batch_free(b);
 		return nullptr;
 	}
-	e = hipMemcpy(b->dtiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice);
+	if (!tiles.empty())
+		e = hipMemcpy(b->dtiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice);
 	if (e == hipSuccess && b->nzero)
 		e = hipMemcpy(b->dzero, zero_ids.data(), zero_ids.size() * sizeof(uint32_t),
 		              hipMemcpyHostToDevice);
@@ -354,11 +459,11 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 	}
 	// ---- graph: Cauchy build, zeroing, queue reset, persistent kernel
 	if (flags & PARHIP_BATCH_GRAPH) {
+		const bool timing = prof_set(false);  // no event nodes in the graph
 		e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
-		if (e == hipSuccess)
-			e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), b->dtiles, b->ntiles, b->dcounter,
-			                 b->dzero, b->nzero, b->coeff_total, b->blocks, b->cap);
+		if (e == hipSuccess) e = batch_enqueue(b, b->cap);
 		hipError_t e2 = hipStreamEndCapture(b->cap, &b->graph);
+		prof_set(timing);
 		if (e == hipSuccess) e = e2;
 		if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
 		if (e != hipSuccess) {
@@ -379,8 +484,7 @@ int parhip_batch_run(parhip_batch* b, void* stream) {
 	if (b->exec) {
 		e = hipGraphLaunch(b->exec, s);
 	} else {
-		e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), b->dtiles, b->ntiles, b->dcounter,
-		                 b->dzero, b->nzero, b->coeff_total, b->blocks, s);
+		e = batch_enqueue(b, s);
 	}
 	return e == hipSuccess ? PARHIP_OK : hip_fail(e, "batch_run");
 }
@@ -404,6 +508,12 @@ int parhip_batch_job_config(const parhip_batch* b, size_t job, int* rt, int* wpt
 	return PARHIP_OK;
 }
 
+int parhip_batch_job_method(const parhip_batch* b, size_t job) {
+	if (!b || job >= b->slot.size()) return fail(PARHIP_ERR_ARG, "batch_job_method: bad job");
+	if (b->slot[job] < 0) return fail(PARHIP_ERR_ARG, "batch_job_method: job %zu is empty", job);
+	return b->cfg[(size_t)b->slot[job]] == BATCH_CFG_MFMA ? PARHIP_METHOD_MFMA : PARHIP_METHOD_TABLE;
+}
+
 void parhip_batch_destroy(parhip_batch* b) { batch_free(b); }
 
 int parhip_autotune_lookup(size_t numInputs, size_t numRecovery, size_t blockSize64, int* rt,
@@ -417,6 +527,13 @@ int parhip_autotune_lookup(size_t numInputs, size_t numRecovery, size_t blockSiz
 	if (ib) *ib = c.ib;
 	if (ns_per_mac) *ns_per_mac = it->second.ns_per_mac;
 	return PARHIP_OK;
+}
+
+int parhip_autotune_lookup_method(size_t numInputs, size_t numRecovery, size_t blockSize64) {
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	auto it = g_tune.find(TuneKey{blockSize64, numRecovery, n_bucket(numInputs)});
+	if (it == g_tune.end()) return fail(PARHIP_ERR_ARG, "autotune: shape not tuned");
+	return it->second.cfg == BATCH_CFG_MFMA ? PARHIP_METHOD_MFMA : PARHIP_METHOD_TABLE;
 }
 
 int parhip_autotune_save(const char* path) {

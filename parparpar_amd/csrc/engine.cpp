@@ -245,6 +245,12 @@ ProfScope::~ProfScope() {
 	t_prof_pending.push_back({name_, a_, b_});
 }
 
+bool prof_set(bool on) {
+	const bool was = t_prof;
+	t_prof = on;
+	return was;
+}
+
 static void prof_collect() {
 	for (auto& r : t_prof_pending) {
 		hipEventSynchronize(r.b);
@@ -339,11 +345,11 @@ size_t plan_workspace_bytes(const Plan& p, uint64_t R, uint64_t W) {
 // GF(2) bit-matrix on MFMA (gf64_mfma.hip): bit-matrix rows of every
 // coefficient prepared on the device, then the MFMA contraction, split-K
 // partials folded as for the table kernel.  Strided slices only.
-static int run_contraction_mfma(const MacArgs& m, bool fp4, hipStream_t s) {
+int plan_mfma(uint64_t N, uint64_t R, uint64_t W, bool fp4, bool overrides, MfmaPlan* p) {
 	// measured best at 1024 x 1 MiB, R = 8 and 32: fp4 4x2 tiles with 4-input
 	// stages; int8 (twice the fragments) 4x1 tiles with 2-input stages
 	int rt = 4, nb = fp4 ? 2 : 1, ib = fp4 ? 4 : 2;
-	if ((t_tune_rt || t_tune_wpt || t_tune_ib) &&
+	if (overrides && (t_tune_rt || t_tune_wpt || t_tune_ib) &&
 	    mfma_supported(t_tune_rt ? t_tune_rt : rt, t_tune_wpt ? t_tune_wpt : nb,
 	                   t_tune_ib ? t_tune_ib : ib, fp4)) {
 		rt = t_tune_rt ? t_tune_rt : rt;
@@ -351,25 +357,31 @@ static int run_contraction_mfma(const MacArgs& m, bool fp4, hipStream_t s) {
 		ib = t_tune_ib ? t_tune_ib : ib;
 	}
 	const uint64_t span = 4ull * 32 * nb;
-	const uint64_t base = ((m.W + span - 1) / span) * ((m.R + rt - 1) / rt);
+	const uint64_t base = ((W + span - 1) / span) * ((R + rt - 1) / rt);
 	const uint64_t target = (uint64_t)device_cus(t_device) * 2;
 	uint64_t groups = base < target ? (target + base - 1) / base : 1;
-	if (t_tune_groups > 0) groups = (uint64_t)t_tune_groups;
-	groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, m.N / 8));
+	if (overrides && t_tune_groups > 0) groups = (uint64_t)t_tune_groups;
+	groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, N / 8));
 	// fp4 counts are fp32 sums of 0/1 products: exact while 64 * inputs < 2^24
-	if (fp4) groups = std::max<uint64_t>(groups, (m.N + (1u << 17) - 1) >> 17);
+	if (fp4) groups = std::max<uint64_t>(groups, (N + (1u << 17) - 1) >> 17);
 	groups = std::min<uint64_t>(groups, 65535);
-	uint64_t per = (m.N + groups - 1) / groups;
-	groups = (m.N + per - 1) / per;
+	uint64_t per = (N + groups - 1) / groups;
+	groups = (N + per - 1) / per;
 	if (fp4 && per >= (1u << 18))
 		return fail(PARHIP_ERR_ARG, "mfma: %llu inputs per group exceed the exact fp32 count",
 		            (unsigned long long)per);
-	const size_t rows_bytes = (size_t)m.R * m.N * 64 * sizeof(uint64_t);
-	const size_t part_bytes =
-	    groups > 1 ? (size_t)xor_reduce_scratch_words(groups, m.R, m.W) * sizeof(uint64_t) : 0;
-	void* ws = nullptr;
-	int rc = dev_alloc(&ws, rows_bytes + part_bytes, s);
-	if (rc) return rc;
+	p->fp4 = fp4;
+	p->rt = rt;
+	p->nb = nb;
+	p->ib = ib;
+	p->groups = groups;
+	p->per = per;
+	p->rows_bytes = (size_t)R * N * 64 * sizeof(uint64_t);
+	p->part_bytes = groups > 1 ? (size_t)xor_reduce_scratch_words(groups, R, W) * sizeof(uint64_t) : 0;
+	return PARHIP_OK;
+}
+
+hipError_t enqueue_mfma(const MacArgs& m, const MfmaPlan& p, void* ws, hipStream_t s) {
 	uint64_t* rows = (uint64_t*)ws;
 	hipError_t e;
 	{
@@ -386,25 +398,33 @@ static int run_contraction_mfma(const MacArgs& m, bool fp4, hipStream_t s) {
 	a.N = m.N;
 	a.R = m.R;
 	a.W = m.W;
-	a.in_per_group = per;
-	a.partial = groups > 1 ? (uint64_t*)((char*)ws + rows_bytes) : nullptr;
+	a.in_per_group = p.per;
+	a.partial = p.groups > 1 ? (uint64_t*)((char*)ws + p.rows_bytes) : nullptr;
 	a.accumulate = m.accumulate;
 	if (e == hipSuccess) {
 		ProfScope ps("mac_mfma", s);
-		e = launch_mac_mfma(a, groups, rt, nb, ib, fp4, s);
+		e = launch_mac_mfma(a, p.groups, p.rt, p.nb, p.ib, p.fp4, s);
 	}
-	if (e == hipSuccess && groups > 1) {
+	if (e == hipSuccess && p.groups > 1) {
 		ProfScope ps("xor_reduce", s);
-		e = launch_xor_reduce(a.partial, groups, m.R, m.W, m.out_base, m.out_stride, nullptr,
+		e = launch_xor_reduce(a.partial, p.groups, m.R, m.W, m.out_base, m.out_stride, nullptr,
 		                      m.accumulate, s);
 	}
+	return e;
+}
+
+static int run_contraction_mfma(const MacArgs& m, bool fp4, hipStream_t s) {
+	MfmaPlan p;
+	int rc = plan_mfma(m.N, m.R, m.W, fp4, true, &p);
+	if (rc) return rc;
+	void* ws = nullptr;
+	rc = dev_alloc(&ws, p.rows_bytes + p.part_bytes, s);
+	if (rc) return rc;
+	hipError_t e = enqueue_mfma(m, p, ws, s);
 	dev_free(ws, s);
 	return e == hipSuccess ? PARHIP_OK : hip_fail(e, "k_mac_mfma");
 }
 
-// 8-bit split tables (gf64_tab8.hip): 4-output tiles for 3..8 outputs, else
-// 2 (one input per LDS batch: 32 or 64 KiB of tables), 8 words per thread on
-// wide slices so that a table is read 8x per build, split-K to fill the device.
 static int run_contraction_tab8(MacArgs a, hipStream_t s) {
 	int rt = (a.R > 2 && a.R <= 8) ? 4 : 2, wpt = a.W >= 2048 ? 8 : (a.W >= 1024 ? 4 : 2), ib = 1;
 	if ((t_tune_rt || t_tune_wpt || t_tune_ib) &&
@@ -455,6 +475,11 @@ int run_contraction(MacArgs a, hipStream_t s) {
 	    !a.out_ptrs)
 		return run_contraction_mfma(a, t_method == PARHIP_METHOD_MFMA, s);
 	if (t_method == PARHIP_METHOD_TABLE8 && !a.in_ptrs && !a.out_ptrs) return run_contraction_tab8(a, s);
+	return run_contraction_tables(a, s);
+}
+
+int run_contraction_tables(MacArgs a, hipStream_t s) {
+	if (a.N == 0 || a.R == 0 || a.W == 0) return PARHIP_OK;
 	Plan p = plan_contraction(a.N, a.R, a.W, t_device);
 	a.in_per_group = p.in_per_group;
 	a.partial = nullptr;

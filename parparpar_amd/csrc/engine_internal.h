@@ -33,12 +33,31 @@ void coeff_cache_clear();
 Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev);
 size_t plan_workspace_bytes(const Plan& p, uint64_t R, uint64_t W);
 int run_contraction(MacArgs a, hipStream_t s);
+// The nibble-table strategy whatever this thread's parhip_set_method says.
+int run_contraction_tables(MacArgs a, hipStream_t s);
+
+// The GF(2) bit-matrix strategy on MFMA: launch shape, split-K groups and the
+// workspace it needs (bit-matrix rows, then split-K partials), and the launch
+// sequence on a caller-provided workspace.  `overrides`: honour this thread's
+// parhip_tune_override.
+struct MfmaPlan {
+	bool fp4;
+	int rt, nb, ib;
+	uint64_t groups, per;
+	size_t rows_bytes, part_bytes;
+};
+int plan_mfma(uint64_t N, uint64_t R, uint64_t W, bool fp4, bool overrides, MfmaPlan* p);
+hipError_t enqueue_mfma(const MacArgs& m, const MfmaPlan& p, void* ws, hipStream_t s);
 
 // PAR2 GF(2^16) side (engine_gf16.cpp)
 int gf16_device_log_exp(const uint16_t** exp_tab, const uint16_t** log_tab);
 int gf16_coeffs_device(uint16_t* d_coeff, const uint16_t* d_input_nums, uint64_t N,
                        const uint16_t* d_exps, uint64_t R, hipStream_t s);
 int run_gf16_contraction(Gf16Args a, hipStream_t s);
+
+// Switch this thread's kernel timing on or off; returns the previous state
+// (launches recorded into a graph are not timed).
+bool prof_set(bool on);
 
 // Brackets a kernel launch with HIP events when timing is enabled
 // (parhip_prof_enable); no-op otherwise.

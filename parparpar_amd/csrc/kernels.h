@@ -27,7 +27,7 @@ struct MacArgs {
 enum BatchCfg : uint16_t {
 	BATCH_CFG_8_4_4 = 0,   // RT=8,  WPT=4, IB=4
 	BATCH_CFG_8_2_4 = 1,   // RT=8,  WPT=2, IB=4
-	BATCH_CFG_UNUSED_2 = 2,
+	BATCH_CFG_MFMA = 2,    // GF(2) bit-matrix on MFMA (fp4): job-level launches, no tiles
 	BATCH_CFG_16_2_2 = 3,  // RT=16, WPT=2, IB=2
 	BATCH_CFG_4_4_4 = 4,   // RT=4,  WPT=4, IB=4
 	BATCH_CFG_COUNT = 5

@@ -95,3 +95,85 @@ def test_batch_skips_empty_jobs(gpu):
     with pytest.raises(P.ParhipError):
         b.job_config(4)
     b.close()
+
+
+def _eligible(shape):
+    n, w, r = shape[:3]
+    return 64 * r <= w
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_mfma_strategy(gpu, oracle, graph):
+    """Jobs whose bit-matrix rows fit (64 R <= words per slice) run on the MFMA
+    strategy when asked; the others stay on the tables; every byte equals the
+    plain computation, replay after replay."""
+    import torch
+    import parparpar_amd as P
+    from parparpar_amd import device
+    from parparpar_amd.batch import Batch
+    rng = np.random.default_rng(404)
+    jobs = _jobs(rng, SHAPES, torch)
+    refs = [device.recovery(x, y.shape[0], fi, fr) for (x, y, fi, fr) in jobs]
+    b = Batch(jobs, graph=graph, autotune=False, mfma=True)
+    for k, s in enumerate(SHAPES):
+        want = P.METHOD_MFMA if _eligible(s) else P.METHOD_TABLE
+        assert b.job_method(k) == want, (k, s)
+    assert 0 < sum(_eligible(s) for s in SHAPES) < len(SHAPES)
+    for rep in range(3):
+        for (_, y, _, _) in jobs:
+            y.fill_(rep + 7)
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        for k, (_, y, _, _) in enumerate(jobs):
+            assert torch.equal(y, refs[k]), "replay %d job %d %s" % (rep, k, SHAPES[k])
+    x, y, fi, fr = jobs[2]
+    cols = np.array([0, 4095, 8191], dtype=np.int64)
+    tc = torch.as_tensor(cols, device="cuda")
+    samp = np.ascontiguousarray(x[:, tc].cpu().numpy().view(np.uint64))
+    exp = oracle.recovery_with_coeff(samp, oracle.cauchy(x.shape[0], y.shape[0], fi, fr))
+    assert np.array_equal(y[:, tc].cpu().numpy().view(np.uint64), exp)
+    b.close()
+
+
+def test_batch_all_mfma_has_no_tiles(gpu):
+    import torch
+    import parparpar_amd as P
+    from parparpar_amd import device
+    from parparpar_amd.batch import Batch
+    rng = np.random.default_rng(405)
+    shapes = [(64, 8192, 8, 0, 64), (33, 1024, 3, 5, 40), (1000, 512, 4, 0, 1000)]
+    jobs = _jobs(rng, shapes, torch)
+    b = Batch(jobs, graph=True, autotune=False, mfma=True)
+    assert b.info()["tiles"] == 0
+    assert all(b.job_method(k) == P.METHOD_MFMA for k in range(len(shapes)))
+    b.run()
+    torch.cuda.synchronize()
+    for (x, y, fi, fr) in jobs:
+        assert torch.equal(y, device.recovery(x, y.shape[0], fi, fr))
+    b.close()
+
+
+def test_autotune_picks_a_strategy_per_shape(gpu):
+    """With autotuning, shapes that qualify for MFMA are timed on both
+    strategies; whichever wins, the cache reports it and the bytes match."""
+    import torch
+    import parparpar_amd as P
+    from parparpar_amd import device
+    from parparpar_amd.batch import Batch
+    L = P.lib()
+    P.check(L.parhip_autotune_clear())
+    rng = np.random.default_rng(406)
+    shapes = [(64, 8192, 8, 0, 64), (77, 1024, 64, 0, 77)]
+    jobs = _jobs(rng, shapes, torch)
+    b = Batch(jobs, graph=True, autotune=True)
+    b.run()
+    torch.cuda.synchronize()
+    for k, (x, y, fi, fr) in enumerate(jobs):
+        assert torch.equal(y, device.recovery(x, y.shape[0], fi, fr))
+        m = b.job_method(k)
+        assert m in (P.METHOD_TABLE, P.METHOD_MFMA)
+        n, w, r = shapes[k][:3]
+        assert L.parhip_autotune_lookup_method(n, r, w) == m
+    assert b.job_method(1) == P.METHOD_TABLE    # 64 R > W: MFMA not offered
+    b.close()
