@@ -80,6 +80,13 @@ int parhip_par2_add_input(parhip_par2 *p, const void *buffer, size_t size, uint1
 /* addInput with explicit per-output coefficients coeffs[numSlices]. */
 int parhip_par2_add_input_coeffs(parhip_par2 *p, const void *buffer, size_t size,
                                  const uint16_t *coeffs, int flush);
+/* Benchmarking, IPAR2ProcBackend::fillInput / dummyInput (gf16/controller.h:172,
+ * :303-305): fill_input copies `buffer` (current slice size bytes) into the next
+ * staging slot and returns 1 once every slot of every staging area holds it
+ * (else 0); dummy_input then adds input `inputNum` over whatever its staging
+ * slot holds, without new data. */
+int parhip_par2_fill_input(parhip_par2 *p, const void *buffer);
+int parhip_par2_dummy_input(parhip_par2 *p, uint16_t inputNum, int flush);
 int parhip_par2_flush(parhip_par2 *p);
 /* endInput: flush and wait until every added input has been processed. */
 int parhip_par2_end_input(parhip_par2 *p);

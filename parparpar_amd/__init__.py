@@ -81,6 +81,8 @@ _SIGS = [
     ("parhip_par2_can_add", _int, [_vp]),
     ("parhip_par2_add_input", _int, [_vp, _vp, _sz, ctypes.c_uint16, _int]),
     ("parhip_par2_add_input_coeffs", _int, [_vp, _vp, _sz, _vp, _int]),
+    ("parhip_par2_fill_input", _int, [_vp, _vp]),
+    ("parhip_par2_dummy_input", _int, [_vp, ctypes.c_uint16, _int]),
     ("parhip_par2_flush", _int, [_vp]),
     ("parhip_par2_end_input", _int, [_vp]),
     ("parhip_par2_get_output", _int, [_vp, ctypes.c_uint, _vp]),

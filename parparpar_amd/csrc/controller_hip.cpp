@@ -219,20 +219,33 @@ void PAR2ProcHIP::dummyInput(uint16_t inputNum, bool flushNow) {
 	if (currentStagingInputs >= inputBatchSize || flushNow) flush();
 }
 
+// Benchmarking (gf16/controller.h:303-305): fill the next staging slot with
+// `buffer` (the current slice size), as PAR2ProcCPU::fillInput does
+// (gf16/controller_cpu.cpp:357-369); true once every slot of every staging
+// area holds it, after which dummyInput() adds inputs over that data.
 bool PAR2ProcHIP::fillInput(const void* buffer) {
-	for (auto& st : staging) {
-		if (st.active) hipEventSynchronize(st.done);
+	Staging& st = staging[currentStagingArea];
+	if (!st.hsrc) {
+		err = "fillInput: no staging memory";
+		return true;
+	}
+	if (st.active) {
+		hipEventSynchronize(st.done);
 		st.active = false;
-		for (unsigned i = 0; i < inputBatchSize; i++) {
-			uint16_t* slot = st.hsrc + (size_t)i * strideWords;
-			std::memcpy(slot, buffer, (words - 1) * 2);
-			slot[words - 1] = 0;
-			uint16_t ck = 0;
-			for (size_t w = 0; w + 1 < words; w++) ck ^= slot[w];
-			slot[words - 1] = ck;
+	}
+	uint16_t* slot = st.hsrc + (size_t)currentStagingInputs * strideWords;
+	const size_t payloadWords = words - 1;
+	std::memcpy(slot, buffer, currentSliceSize);
+	std::memset(reinterpret_cast<char*>(slot) + currentSliceSize, 0, words * 2 - currentSliceSize);
+	slot[payloadWords] = slice_checksum(slot, payloadWords);
+	if (++currentStagingInputs == inputBatchSize) {
+		currentStagingInputs = 0;
+		if (++currentStagingArea == staging.size()) {
+			currentStagingArea = 0;
+			return true;  // all filled
 		}
 	}
-	return true;
+	return false;
 }
 
 void PAR2ProcHIP::flush() {
@@ -388,6 +401,19 @@ int parhip_par2_add_input_coeffs(parhip_par2* h, const void* buffer, size_t size
 		return fail(PARHIP_ERR_ARG, "par2_add_input_coeffs: null pointer");
 	h->be->addInput(buffer, size, coeffs, flush != 0).get();
 	return be_status(h);
+}
+
+int parhip_par2_dummy_input(parhip_par2* h, uint16_t inputNum, int flush) {
+	if (!h) return fail(PARHIP_ERR_ARG, "par2: null handle");
+	h->be->dummyInput(inputNum, flush != 0);
+	return be_status(h);
+}
+
+int parhip_par2_fill_input(parhip_par2* h, const void* buffer) {
+	if (!h || !buffer) return fail(PARHIP_ERR_ARG, "par2_fill_input: null pointer");
+	const bool all = h->be->fillInput(buffer);
+	const int rc = be_status(h);
+	return rc ? rc : (all ? 1 : 0);
 }
 
 int parhip_par2_flush(parhip_par2* h) {

@@ -61,6 +61,21 @@ class GfProc:
         check(lib().parhip_par2_add_input_coeffs(self._h, b.ctypes.data, b.size, c.ctypes.data,
                                                  1 if flush else 0))
 
+    def fill_input(self, buf):
+        """Benchmarking: copy `buf` (the current slice size) into the next staging
+        slot; True once every staging slot holds it."""
+        b = np.ascontiguousarray(buf).view(np.uint8)
+        if b.size < self.slice_size:
+            raise ValueError("fill_input needs the current slice size (%d bytes)" % self.slice_size)
+        rc = lib().parhip_par2_fill_input(self._h, b.ctypes.data)
+        if rc < 0:
+            check(rc)
+        return rc == 1
+
+    def dummy_input(self, input_num, flush=False):
+        """Benchmarking: add input `input_num` over the data its staging slot holds."""
+        check(lib().parhip_par2_dummy_input(self._h, input_num, 1 if flush else 0))
+
     def flush(self):
         check(lib().parhip_par2_flush(self._h))
 

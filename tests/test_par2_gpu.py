@@ -134,3 +134,32 @@ def test_device_muladd_columns(gpu, oracle, n, words, r):
     assert np.array_equal(got, ref)
     del x, out
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("s,batch,nin", [(20000, 4, 11), (19999, 3, 6), (2, 2, 5)])
+def test_fill_then_dummy_inputs(gpu, oracle, s, batch, nin):
+    """The benchmarking pair of the backend contract (gf16/controller.h:303-305):
+    fill_input fills one staging slot per call until every slot of both staging
+    areas holds the buffer; dummy_input then adds inputs over that data, so
+    recovery e = XOR_i coeff(i, e) * buffer, checksum valid."""
+    from parparpar_amd.par2 import GfProc
+    rng = np.random.default_rng(s + batch)
+    buf = rng.integers(0, 256, size=s, dtype=np.uint8)
+    exps = [0, 1, 77, 65534]
+    inums = rng.integers(0, 32768, size=nin).tolist()
+    gp = GfProc(s, input_batch=batch)
+    gp.set_recovery_slices(exps)
+    calls = 1
+    while not gp.fill_input(buf):
+        calls += 1
+        assert calls <= 2 * batch
+    assert calls == 2 * batch            # one slot per call, two staging areas
+    for i in inums:
+        gp.dummy_input(i)
+    gp.end()
+    ref = oracle.par2_create([buf] * nin, inums, exps, s)
+    for e in range(len(exps)):
+        valid, out = gp.get(e)
+        assert valid, "checksum invalid for output %d" % e
+        assert np.array_equal(out, ref[e]), e
+    gp.close()
