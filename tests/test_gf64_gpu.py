@@ -323,3 +323,34 @@ def test_device_strided_and_unaligned(gpu, oracle):
     device.recovery(view, r, 0, n, out=out)
     host = np.ascontiguousarray(view.cpu().numpy().view(np.uint64))
     assert np.array_equal(out.cpu().numpy().view(np.uint64), oracle.recovery(host, r, 0, n))
+
+
+@pytest.mark.parametrize("shift_in,shift_out,acc", [(1, 3, False), (4, 0, True), (7, 5, True)])
+def test_host_buffers_not_8_byte_aligned(gpu, oracle, shift_in, shift_out, acc):
+    """Host buffers at any byte address: the addon bounce-copies 8-byte
+    misaligned Buffers (src/gf64_addon.cc:1466-1505); the library takes them as
+    they are. Small (one copy) and multi-chunk (streamed) sizes."""
+    import parparpar_amd as P
+    from parparpar_amd import binding
+    rng = np.random.default_rng(shift_in * 10 + shift_out)
+    L = P.lib()
+    for n, w, r in [(5, 129, 3), (40, 8192, 6)]:
+        inp = rand_words(rng, n, w)
+        init = rand_words(rng, r, w)
+        raw_in = np.zeros(n * w * 8 + 16, dtype=np.uint8)
+        raw_out = np.zeros(r * w * 8 + 16, dtype=np.uint8)
+        vin = raw_in[shift_in:shift_in + n * w * 8]
+        vout = raw_out[shift_out:shift_out + r * w * 8]
+        vin[:] = inp.view(np.uint8).reshape(-1)
+        vout[:] = init.view(np.uint8).reshape(-1)
+        assert vin.ctypes.data % 8 == shift_in % 8
+        P.check(L.parhip_set_stream_chunk(1 << 20))   # the 2.5 MiB case streams in chunks
+        try:
+            binding.compute_recovery(vin, vout, n, r, w * 8, 0, n, 0, accumulate=acc)
+        finally:
+            P.check(L.parhip_set_stream_chunk(0))
+        expect = init.copy() if acc else np.zeros((r, w), dtype=np.uint64)
+        oracle.recovery_with_coeff(inp, oracle.cauchy(n, r, 0, n), expect, acc)
+        got = np.frombuffer(vout.tobytes(), dtype=np.uint64).reshape(r, w)
+        assert np.array_equal(got, expect), (n, w, r)
+        assert not raw_out[:shift_out].any() and not raw_out[shift_out + r * w * 8:].any()
