@@ -354,3 +354,42 @@ def test_host_buffers_not_8_byte_aligned(gpu, oracle, shift_in, shift_out, acc):
         got = np.frombuffer(vout.tobytes(), dtype=np.uint64).reshape(r, w)
         assert np.array_equal(got, expect), (n, w, r)
         assert not raw_out[:shift_out].any() and not raw_out[shift_out + r * w * 8:].any()
+
+
+# Compute scenarios of test/par3-native-edge.js (3, 4, 9-12, 15, 16):
+# (numInputs, numRecovery, blockSize, firstInput, firstRecovery, numThreads)
+EDGE = [(1, 1, 8, 0, 2, 1), (2, 1, 8, 0, 2, 0), (4, 3, 64, 0, 4, 9999), (4, 3, 64, 0, 4, 0),
+        (5, 100, 256, 0, 5, 1), (6, 4, 128, 1000, 2000, 2), (10, 100, 512, 0, 10, 0)]
+
+
+@pytest.mark.parametrize("n,r,bs,fi,fr,threads", EDGE)
+def test_reference_edge_scenarios(gpu, oracle, n, r, bs, fi, fr, threads):
+    from parparpar_amd import binding
+    rng = np.random.default_rng(n * 131 + r * 7 + bs)
+    w = bs // 8
+    inp = rand_words(rng, n, w)
+    if (n, r, bs) == (1, 1, 8):
+        inp[0, 0] = np.uint64(0xDEADBEEFCAFEBABE)     # scenario 3's word
+    out = np.zeros((r, w), dtype=np.uint64)
+    binding.compute_recovery(inp, out, n, r, bs, fi, fr, threads)
+    assert np.array_equal(out, oracle.recovery_with_coeff(inp, oracle.cauchy(n, r, fi, fr)))
+    assert out.any()
+    if n == 1:
+        assert not np.array_equal(out[0], inp[0])     # Cauchy coefficient 1/(0 ^ 2) != 1
+    again = np.zeros_like(out)
+    binding.compute_recovery(inp, again, n, r, bs, fi, fr, threads)   # scenario 16
+    assert np.array_equal(out, again)
+
+
+def test_reference_edge_64mib_blocks(gpu, oracle):
+    """Scenario 7: 2 inputs x 2 recovery of 64 MiB blocks (host buffers, streamed)."""
+    from parparpar_amd import binding
+    rng = np.random.default_rng(7)
+    w = (64 << 20) // 8
+    inp = np.frombuffer(rng.bytes(2 * w * 8), dtype=np.uint64).reshape(2, w).copy()
+    out = np.zeros((2, w), dtype=np.uint64)
+    binding.compute_recovery(inp, out, 2, 2, 64 << 20, 0, 2, 0)
+    cols = np.unique(np.concatenate([[0, 1, w // 2, w - 1], rng.integers(0, w, size=60)]))
+    exp = oracle.recovery_with_coeff(np.ascontiguousarray(inp[:, cols]), oracle.cauchy(2, 2, 0, 2))
+    assert np.array_equal(out[:, cols], exp)
+    assert out[0, 0] != 0
