@@ -38,6 +38,18 @@ def test_kat(gpu):
     assert int(out[0]) == 0x7ffffffffffffffa
 
 
+def test_native_fidelity_cauchy_consistency(gpu, oracle):
+    # test/par3-native-fidelity.js:302-360: inputs [1, 2, 3], R = 2, points (0, 3)
+    from parparpar_amd import binding
+    inp = np.array([1, 2, 3], dtype=np.uint64)
+    out = np.zeros(2, dtype=np.uint64)
+    binding.compute_recovery(inp, out, 3, 2, 8, 0, 3)
+    co = oracle.cauchy(3, 2, 0, 3)
+    expect = [oracle.gf64_mul(1, int(co[r, 0])) ^ oracle.gf64_mul(2, int(co[r, 1])) ^
+              oracle.gf64_mul(3, int(co[r, 2])) for r in range(2)]
+    assert [int(x) for x in out] == expect
+
+
 def test_config1_64x64k_r8(gpu, oracle):
     # 64 input slices x 64 KiB of seeded random bytes, R=8, points (0, 64)
     from parparpar_amd import binding

@@ -52,6 +52,48 @@ def test_kat_native_fidelity(oracle):
     assert int(out[0, 0]) == 0x7ffffffffffffffa
 
 
+def _peasant(a, b):
+    """Textbook GF(2^64) product: shift-and-add over b's bits, reducing by
+    x^64 = x^4 + x^3 + x + 1 at every overflow."""
+    r = 0
+    for _ in range(64):
+        if b & 1:
+            r ^= a
+        b >>= 1
+        a <<= 1
+        if a >> 64:
+            a = (a & (2 ** 64 - 1)) ^ 0x1B
+    return r
+
+
+def _inverse(a):
+    """a^(2^64 - 2) by square-and-multiply over _peasant."""
+    r, e = 1, 2 ** 64 - 2
+    while e:
+        if e & 1:
+            r = _peasant(r, a)
+        a = _peasant(a, a)
+        e >>= 1
+    return r
+
+
+def test_native_fidelity_cauchy_consistency(oracle):
+    """test/par3-native-fidelity.js:302-360 (Section C): 3 inputs [1, 2, 3] x 2
+    recovery, blockSize 8, points (0, 3); the matrix inv(c ^ (3 + r)) and the
+    multiply-accumulate computed independently in Python."""
+    inputs = [1, 2, 3]
+    expected = []
+    for r in range(2):
+        acc = 0
+        for c, v in enumerate(inputs):
+            acc ^= _peasant(v, _inverse(c ^ (3 + r)))
+        expected.append(acc)
+    got = oracle.recovery(np.array([[v] for v in inputs], dtype=np.uint64), 2, 0, 3)
+    assert [int(x) for x in got[:, 0]] == expected
+    # the KAT of Section A through the same independent path
+    assert _peasant(1, _inverse(0 ^ 2)) ^ _peasant(2, _inverse(1 ^ 2)) == 0x7ffffffffffffffa
+
+
 def test_kat_mul(oracle):
     # test/gf64-js.test.js:286-296
     assert oracle.gf64_mul(2, 3) == 6
