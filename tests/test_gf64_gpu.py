@@ -405,3 +405,20 @@ def test_reference_edge_64mib_blocks(gpu, oracle):
     exp = oracle.recovery_with_coeff(np.ascontiguousarray(inp[:, cols]), oracle.cauchy(2, 2, 0, 2))
     assert np.array_equal(out[:, cols], exp)
     assert out[0, 0] != 0
+
+
+@pytest.mark.parametrize("ln", [0, 1, 2, 3, 4, 5, 8, 16, 17, 33])
+def test_region_mul_arr_reference_grid(gpu, oracle, ln):
+    """gf64/test/test_gf64_region_mularr.c:50-85: every length in its list with
+    n_coeff in {1, 2, 3, 4, 7}; out[i] = in[i] * coeff[i % n_coeff] (cycling, not
+    a dot product), and nothing past len is written."""
+    from parparpar_amd import binding
+    rng = np.random.default_rng(0xD3D3 + ln)
+    x = rand_words(rng, max(ln, 1))
+    for nc in (1, 2, 3, 4, 7):
+        co = rand_words(rng, nc)
+        out = np.full(ln + 2, 0xAAAAAAAAAAAAAAAA, dtype=np.uint64)
+        binding.mul_arr(out, x, co, ln, nc)
+        expect = [oracle.gf64_mul(int(x[i]), int(co[i % nc])) for i in range(ln)]
+        assert [int(v) for v in out[:ln]] == expect, (ln, nc)
+        assert (out[ln:] == np.uint64(0xAAAAAAAAAAAAAAAA)).all()
