@@ -422,3 +422,31 @@ def test_region_mul_arr_reference_grid(gpu, oracle, ln):
         expect = [oracle.gf64_mul(int(x[i]), int(co[i % nc])) for i in range(ln)]
         assert [int(v) for v in out[:ln]] == expect, (ln, nc)
         assert (out[ln:] == np.uint64(0xAAAAAAAAAAAAAAAA)).all()
+
+
+@pytest.mark.parametrize("G", [1, 2, 4, 8, 32, 64, 248])
+def test_repair_rhs_kernel_grid(gpu, oracle, G):
+    """test/par3-repair-rhs-regression.js:52-128: coupled_muladd_arr per row and
+    two_d_muladd_arr for all rows, over G in its list, K in {1, 2, 4, 8} and
+    len in {1, 7, 8, 32, 64, 128} words, against the oracle's contraction; plus
+    its negative trap (one corrupted coefficient changes the result)."""
+    from parparpar_amd import binding
+    enc = binding.Gf64Encoder()
+    rng = np.random.default_rng(90 + G)
+    for K in (1, 2, 4, 8):
+        for ln in (1, 7, 8, 32, 64, 128):
+            ins = rand_words(rng, G, ln)
+            c2d = rand_words(rng, K, G)
+            ref = oracle.recovery_with_coeff(ins, c2d)
+            for k in range(K):
+                o = np.zeros(ln, dtype=np.uint64)
+                enc.coupled_muladd_arr(o, list(ins), np.ascontiguousarray(c2d[k]), ln)
+                assert np.array_equal(o, ref[k]), ("coupled", G, K, ln, k)
+            outs = [np.zeros(ln, dtype=np.uint64) for _ in range(K)]
+            enc.two_d_muladd_arr(outs, list(ins), c2d, G, ln)
+            assert np.array_equal(np.stack(outs), ref), ("2d", G, K, ln)
+    bad = c2d.copy()
+    bad[0, 0] ^= np.uint64(1)
+    outs = [np.zeros(ln, dtype=np.uint64) for _ in range(K)]
+    enc.two_d_muladd_arr(outs, list(ins), bad, G, ln)
+    assert not np.array_equal(np.stack(outs), ref)
