@@ -450,3 +450,37 @@ def test_repair_rhs_kernel_grid(gpu, oracle, G):
     outs = [np.zeros(ln, dtype=np.uint64) for _ in range(K)]
     enc.two_d_muladd_arr(outs, list(ins), bad, G, ln)
     assert not np.array_equal(np.stack(outs), ref)
+
+
+def test_past_4gib_boundary(gpu, oracle):
+    """test/par3-repair-4100-hash.js guards offsets past 4 GiB: 4100 slices of
+    1 MiB (4.0 GiB + 4 MiB) on the device and, streamed, from host memory; the
+    columns sampled against the oracle include every input, those past the
+    boundary too, and a one-word change in input 4099 shows up in its column."""
+    import torch
+    from parparpar_amd import binding, device
+    n, w, r = 4100, (1 << 20) // 8, 2
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4100)
+    d_in = torch.randint(-2 ** 63, 2 ** 63 - 1, (n, w), dtype=torch.int64, device="cuda", generator=g)
+    d_out = device.recovery(d_in, r, 0, n)
+    torch.cuda.synchronize()
+    cols = np.array([0, 1, w // 2, w - 2, w - 1], dtype=np.int64)
+    tc = torch.as_tensor(cols, device="cuda")
+    samp = np.ascontiguousarray(d_in[:, tc].cpu().numpy().view(np.uint64))
+    co = oracle.cauchy(n, r, 0, n)
+    exp = oracle.recovery_with_coeff(samp, co)
+    assert np.array_equal(d_out[:, tc].cpu().numpy().view(np.uint64), exp)
+    # host entry over the same bytes (a 4.0 GiB pageable buffer, streamed)
+    host = d_in.cpu().numpy().view(np.uint64)
+    out = np.zeros((r, w), dtype=np.uint64)
+    binding.compute_recovery(host, out, n, r, w * 8, 0, n)
+    assert np.array_equal(out, d_out.cpu().numpy().view(np.uint64))
+    # the last input sits past the boundary: flipping one of its words moves its column only
+    d_in[n - 1, w - 1] ^= 1
+    d_out2 = device.recovery(d_in, r, 0, n)
+    torch.cuda.synchronize()
+    diff = (d_out2 != d_out).any(dim=0).nonzero().flatten().cpu().tolist()
+    assert diff == [w - 1]
+    del d_in, d_out, d_out2, host
+    torch.cuda.empty_cache()
