@@ -21,8 +21,11 @@ def rand_words(rng, *shape):
     return rng.integers(0, 2 ** 64, size=shape, dtype=np.uint64)
 
 
+# the last two are test/par3-barycentric-parallel-parity.js:167-168 (64 x 64 x 64 words at
+# points (1000, 100000); 32768 inputs of 8 words, R = 256)
 @pytest.mark.parametrize("n,w,r,fi,fr", [(5, 128, 2, 0, 5), (40, 64, 33, 0, 40), (17, 96, 48, 100, 3),
-                                         (64, 512, 8, 0, 64)])
+                                         (64, 512, 8, 0, 64), (64, 64, 64, 1000, 100000),
+                                         (32768, 8, 256, 0, 32768)])
 def test_barycentric(gpu, oracle, n, w, r, fi, fr):
     from parparpar_amd import binding
     rng = np.random.default_rng(n * r)
