@@ -57,6 +57,33 @@ def test_fenger(gpu, oracle):
         binding.compute_recovery_fenger(keep, keep, -1, 2, 64, 0, 0)
 
 
+# test/par3-fenger-padded-engine.js:103-127: odd and power-of-two N and R at
+# points (0x10000, 0x1000000), N = 1, and the two overlapping point ranges
+# (recovery points inside the input range: zero denominators take the 0 -> 1 rule)
+@pytest.mark.parametrize("n,r,fi,fr", [(1000, 512, 0x10000, 0x1000000), (1000, 600, 0x10000, 0x1000000),
+                                       (1024, 600, 0x10000, 0x1000000), (1, 512, 0x10000, 0x1000000),
+                                       (1000, 512, 0x10000, 66300), (1024, 512, 0x10000, 66400)])
+def test_fenger_padded_engine_cases(gpu, oracle, n, r, fi, fr):
+    from parparpar_amd import binding
+    rng = np.random.default_rng(n + r + fr)
+    w = 128 // 8
+    inp = rand_words(rng, n, w)
+    out = np.zeros((r, w), dtype=np.uint64)
+    binding.compute_recovery_fenger(inp, out, n, r, 128, fi, fr)
+    assert np.array_equal(out, oracle.recovery(inp, r, fi, fr))
+
+
+def test_fenger_rejects_impossible_sizes(gpu):
+    """The same file's guards: blockSize 2^32 + 8 and numInputs 2^63 are refused
+    cleanly, before any work."""
+    from parparpar_amd import binding
+    b = np.zeros(1, dtype=np.uint64)
+    with pytest.raises(ValueError):
+        binding.compute_recovery_fenger(b, b.copy(), 1, 1, 2 ** 32 + 8, 0x10000, 0x1000000)
+    with pytest.raises((ValueError, OverflowError)):
+        binding.compute_recovery_fenger(b, b.copy(), 2 ** 63, 2, 8, 0x10000, 0x1000000)
+
+
 def test_barycentric_errors(gpu):
     from parparpar_amd import binding
     buf = np.zeros(16, dtype=np.uint64)
