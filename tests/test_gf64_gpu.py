@@ -484,3 +484,26 @@ def test_past_4gib_boundary(gpu, oracle):
     assert diff == [w - 1]
     del d_in, d_out, d_out2, host
     torch.cuda.empty_cache()
+
+
+def test_mul_arr_semantics_reference_cases(gpu, oracle):
+    """gf64/test/test.c:292-346: n_coeff = 1 is a scalar multiply, n_coeff > 1
+    cycles, coefficient 0 zeroes, 1 is the identity, and in == out works."""
+    from parparpar_amd import binding
+    x = np.array([1, 2, 3, 4], dtype=np.uint64)
+    m = oracle.gf64_mul
+    out = np.zeros(4, dtype=np.uint64)
+    binding.mul_arr(out, x, np.array([0xDEADBEEF12345678], dtype=np.uint64), 4, 1)
+    assert [int(v) for v in out] == [m(int(v), 0xDEADBEEF12345678) for v in x]
+    co = np.array([0xAA00AA00AA00AA00, 0xBB00BB00BB00BB00], dtype=np.uint64)
+    binding.mul_arr(out, x, co, 4, 2)
+    assert [int(v) for v in out] == [m(int(x[i]), int(co[i % 2])) for i in range(4)]
+    out[:] = 0xDEADBEEF
+    binding.mul_arr(out, x, np.array([0], dtype=np.uint64), 4, 1)
+    assert not out.any()
+    binding.mul_arr(out, x, np.array([1], dtype=np.uint64), 4, 1)
+    assert np.array_equal(out, x)
+    buf = x.copy()
+    co = np.array([0xCC00CC00CC00CC00, 0xDD00DD00DD00DD00], dtype=np.uint64)
+    binding.mul_arr(buf, buf, co, 4, 2)
+    assert [int(v) for v in buf] == [m(int(x[i]), int(co[i % 2])) for i in range(4)]
