@@ -1,6 +1,8 @@
 // controller_hip.cpp -- PAR2ProcHIP (see controller_hip.h) and its C ABI.
 #include "controller_hip.h"
 
+#include <emmintrin.h>
+
 #include <algorithm>
 #include <cstring>
 
@@ -13,38 +15,45 @@ namespace parhip {
 
 int run_gf16_contraction(Gf16Args a, hipStream_t s);
 
-// Checksum word of a staged slice: sum over words w of x^(n'-1-w) * word_w in
-// GF(2^16) (n' = word count rounded up to a multiple of 4), the positional
-// checksum of the reference's staging (gf16/gf16_checksum_generic.h: Horner
-// steps of a multiply by x and an XOR per block).  Linear over GF(2^16), so the
-// muladd carries it: cksum(sum_i c_i * in_i) = sum_i c_i * cksum(in_i).  Unlike
-// a plain XOR it also catches words that land in the wrong place.  Computed as
-// four interleaved 16-bit lanes of a 64-bit word, each stepped by x^4 per
-// 4-word chunk, then folded as x^3 L0 + x^2 L1 + x L2 + L3.
-static inline uint64_t lanes_mulx4(uint64_t v) {
-	const uint64_t t = (v >> 12) & 0x000F000F000F000Full;  // x^16 = x^12 + x^3 + x + 1
-	return ((v << 4) & 0xFFF0FFF0FFF0FFF0ull) ^ (t << 12) ^ (t << 3) ^ (t << 1) ^ t;
-}
-
-static inline uint16_t mulx16(uint16_t a) {
-	return (uint16_t)((a << 1) ^ ((a & 0x8000u) ? 0x100Bu : 0u));
+// Checksum word of a staged slice of n words, in m = ceil(n/8) chunks of 8
+// (the last zero-padded): word w = 8j + k carries the weight x^(m(7-k) + m-1-j)
+// in GF(2^16), every word a different power of x below x^(8m).  It is the
+// positional checksum of the reference's staging (gf16/gf16_checksum_generic.h:
+// per block, multiply the running checksum by x and XOR the block in) with the
+// 8 lanes of a block folded into one word.  Linear over GF(2^16), so the muladd
+// carries it (cksum(sum_i c_i in_i) = sum_i c_i cksum(in_i)), and words that land
+// in the wrong place change it.  Computed with SSE2 over 8 lanes, the chunks
+// split into 4 runs stepped independently (a step is x per chunk) and joined
+// with one multiply per lane.
+static inline __m128i ck_step(__m128i a, __m128i v) {
+	const __m128i carry = _mm_and_si128(_mm_srai_epi16(a, 15), _mm_set1_epi16(0x100B));
+	return _mm_xor_si128(_mm_xor_si128(_mm_slli_epi16(a, 1), carry), v);
 }
 
 static uint16_t slice_checksum(const uint16_t* p, size_t n) {
-	uint64_t lanes = 0;
-	size_t w = 0;
-	for (; w + 4 <= n; w += 4) {
-		uint64_t v;
-		std::memcpy(&v, p + w, 8);
-		lanes = lanes_mulx4(lanes) ^ v;
+	const Gf16HostTables& t = gf16_host_tables();
+	const size_t m = (n + 7) / 8, full = n / 8, L = full / 4;
+	__m128i a[4] = {_mm_setzero_si128(), _mm_setzero_si128(), _mm_setzero_si128(),
+	                _mm_setzero_si128()};
+	for (size_t j = 0; j < L; j++)
+		for (int r = 0; r < 4; r++)
+			a[r] = ck_step(a[r], _mm_loadu_si128(reinterpret_cast<const __m128i*>(p + 8 * (r * L + j))));
+	for (size_t j = 4 * L; j < full; j++)
+		a[3] = ck_step(a[3], _mm_loadu_si128(reinterpret_cast<const __m128i*>(p + 8 * j)));
+	if (full < m) {
+		uint16_t tail[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+		std::memcpy(tail, p + 8 * full, (n - 8 * full) * 2);
+		a[3] = ck_step(a[3], _mm_loadu_si128(reinterpret_cast<const __m128i*>(tail)));
 	}
-	if (w < n) {
-		uint64_t v = 0;
-		std::memcpy(&v, p + w, (n - w) * 2);
-		lanes = lanes_mulx4(lanes) ^ v;
+	uint16_t run[4][8], lane[8];
+	for (int r = 0; r < 4; r++) _mm_storeu_si128(reinterpret_cast<__m128i*>(run[r]), a[r]);
+	for (int k = 0; k < 8; k++) lane[k] = run[3][k];
+	for (int r = 0; r < 3; r++) {  // run r ends (r+1)L chunks in: x^(chunks after it)
+		const uint16_t f = t.exp[(m - (uint64_t)(r + 1) * L) % 65535u];
+		for (int k = 0; k < 8; k++) lane[k] ^= gf16_host_mul(run[r][k], f);
 	}
 	uint16_t ck = 0;
-	for (int l = 0; l < 4; l++) ck = (uint16_t)(mulx16(ck) ^ (uint16_t)(lanes >> (16 * l)));
+	for (int k = 0; k < 8; k++) ck ^= gf16_host_mul(lane[k], t.exp[((uint64_t)m * (7 - k)) % 65535u]);
 	return ck;
 }
 
@@ -176,8 +185,17 @@ void PAR2ProcHIP::waitForAdd() {
 }
 
 void PAR2ProcHIP::setCoeffs(Staging& st, unsigned idx, uint16_t inputNum) {
-	for (size_t e = 0; e < outputExponents.size(); e++)
-		st.hcoef[e * inputBatchSize + idx] = gf16_host_coeff(inputNum, outputExponents[e]);
+	// gf16_host_coeff with the modulo 65535 folded by hand: the product of two
+	// values below 2^16 fits 32 bits, and x mod 65535 = (x & 0xFFFF) + (x >> 16)
+	// up to one subtraction
+	const Gf16HostTables& t = gf16_host_tables();
+	const uint32_t li = t.inlog[inputNum & 32767u];
+	for (size_t e = 0; e < outputExponents.size(); e++) {
+		uint32_t x = li * outputExponents[e];
+		x = (x & 0xFFFFu) + (x >> 16);
+		if (x >= 65535u) x -= 65535u;
+		st.hcoef[e * inputBatchSize + idx] = t.exp[x];
+	}
 }
 
 void PAR2ProcHIP::setCoeffs(Staging& st, unsigned idx, const uint16_t* coeffs) {
