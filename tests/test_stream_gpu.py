@@ -109,3 +109,20 @@ def test_create_streaming_errors(gpu, tmp_path):
         binding.par3_create_streaming(str(f), {"recoverySlices": 2, "blockSize": 1020})
     with pytest.raises(ParhipError, match="positive"):
         binding.par3_create_streaming(str(f), {"recoverySlices": 0, "blockSize": 1024})
+
+
+@pytest.mark.parametrize("mmap", [False, True])
+def test_create_streaming_short_file_leg5(chunk, oracle, tmp_path, mmap):
+    """test/par3-create-streaming-parity.js:321-330 (leg 5): a 1000-byte source,
+    bytes (7i + 3) mod 256, 4096-byte blocks, 4 recovery slices: the one block is
+    zero-padded, read or mapped."""
+    from parparpar_amd import binding
+    chunk(0)
+    data = bytes((7 * i + 3) & 0xFF for i in range(1000))
+    path = tmp_path / "leg5.bin"
+    path.write_bytes(data)
+    res = binding.par3_create_streaming(str(path), {"recoverySlices": 4, "blockSize": 4096,
+                                                    "firstInput": 0, "firstRecovery": 1,
+                                                    "useMmap": mmap})
+    padded = np.frombuffer(data + bytes(4096 - 1000), dtype=np.uint64).reshape(1, 512)
+    assert res["recoveryBuffer"] == oracle.recovery(padded, 4, 0, 1).tobytes()
