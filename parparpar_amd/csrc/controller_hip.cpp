@@ -9,6 +9,7 @@
 #include "../../include/parhip.h"
 #include "engine_internal.h"
 #include "gf16_host.h"
+#include "ingest.h"
 #include "kernels.h"
 
 namespace parhip {
@@ -30,31 +31,100 @@ static inline __m128i ck_step(__m128i a, __m128i v) {
 	return _mm_xor_si128(_mm_xor_si128(_mm_slli_epi16(a, 1), carry), v);
 }
 
-static uint16_t slice_checksum(const uint16_t* p, size_t n) {
-	const Gf16HostTables& t = gf16_host_tables();
-	const size_t m = (n + 7) / 8, full = n / 8, L = full / 4;
-	__m128i a[4] = {_mm_setzero_si128(), _mm_setzero_si128(), _mm_setzero_si128(),
-	                _mm_setzero_si128()};
-	for (size_t j = 0; j < L; j++)
-		for (int r = 0; r < 4; r++)
-			a[r] = ck_step(a[r], _mm_loadu_si128(reinterpret_cast<const __m128i*>(p + 8 * (r * L + j))));
-	for (size_t j = 4 * L; j < full; j++)
-		a[3] = ck_step(a[3], _mm_loadu_si128(reinterpret_cast<const __m128i*>(p + 8 * j)));
-	if (full < m) {
+// Chunk range [c0, c1) of run r out of `runs` over n words: the full chunks
+// split evenly; the last run also takes the zero-padded partial chunk.
+struct CkRun {
+	size_t c0, c1;
+	bool tail;
+};
+static CkRun ck_run(size_t n, size_t runs, size_t r) {
+	const size_t full = n / 8;
+	CkRun k{full * r / runs, full * (r + 1) / runs, r + 1 == runs && (n % 8) != 0};
+	return k;
+}
+
+// One run's Horner value (lanes), x per chunk.
+static __m128i ck_run_value(const uint16_t* p, size_t n, const CkRun& k) {
+	__m128i a = _mm_setzero_si128();
+	for (size_t j = k.c0; j < k.c1; j++)
+		a = ck_step(a, _mm_loadu_si128(reinterpret_cast<const __m128i*>(p + 8 * j)));
+	if (k.tail) {
 		uint16_t tail[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-		std::memcpy(tail, p + 8 * full, (n - 8 * full) * 2);
-		a[3] = ck_step(a[3], _mm_loadu_si128(reinterpret_cast<const __m128i*>(tail)));
+		std::memcpy(tail, p + 8 * k.c1, (n - 8 * k.c1) * 2);
+		a = ck_step(a, _mm_loadu_si128(reinterpret_cast<const __m128i*>(tail)));
 	}
-	uint16_t run[4][8], lane[8];
-	for (int r = 0; r < 4; r++) _mm_storeu_si128(reinterpret_cast<__m128i*>(run[r]), a[r]);
-	for (int k = 0; k < 8; k++) lane[k] = run[3][k];
-	for (int r = 0; r < 3; r++) {  // run r ends (r+1)L chunks in: x^(chunks after it)
-		const uint16_t f = t.exp[(m - (uint64_t)(r + 1) * L) % 65535u];
-		for (int k = 0; k < 8; k++) lane[k] ^= gf16_host_mul(run[r][k], f);
+	return a;
+}
+
+// Join run values: run r is followed by x^(chunks after it); then fold the 8
+// lanes with x^(m(7-k)).
+static uint16_t ck_join(const __m128i* v, size_t runs, size_t n) {
+	const Gf16HostTables& t = gf16_host_tables();
+	const size_t m = (n + 7) / 8;
+	uint16_t lane[8] = {0, 0, 0, 0, 0, 0, 0, 0}, run[8];
+	for (size_t r = 0; r < runs; r++) {
+		_mm_storeu_si128(reinterpret_cast<__m128i*>(run), v[r]);
+		const CkRun k = ck_run(n, runs, r);
+		const uint64_t after = m - (k.c1 + (k.tail ? 1 : 0));
+		const uint16_t f = t.exp[after % 65535u];
+		for (int q = 0; q < 8; q++) lane[q] ^= gf16_host_mul(run[q], f);
 	}
 	uint16_t ck = 0;
-	for (int k = 0; k < 8; k++) ck ^= gf16_host_mul(lane[k], t.exp[((uint64_t)m * (7 - k)) % 65535u]);
+	for (int q = 0; q < 8; q++) ck ^= gf16_host_mul(lane[q], t.exp[((uint64_t)m * (7 - q)) % 65535u]);
 	return ck;
+}
+
+// The checksum of n words with four runs stepped together on this thread
+// (independent dependency chains).
+static uint16_t slice_checksum(const uint16_t* p, size_t n) {
+	constexpr size_t R = 4;
+	CkRun k[R];
+	for (size_t r = 0; r < R; r++) k[r] = ck_run(n, R, r);
+	__m128i a[R];
+	for (size_t r = 0; r < R; r++) a[r] = _mm_setzero_si128();
+	const size_t L = k[0].c1 - k[0].c0;  // runs 0..2 are L chunks, run 3 L or more
+	for (size_t j = 0; j < L; j++)
+		for (size_t r = 0; r < R; r++)
+			a[r] = ck_step(a[r], _mm_loadu_si128(reinterpret_cast<const __m128i*>(p + 8 * (k[r].c0 + j))));
+	for (size_t r = 0; r < R; r++) {  // the chunks of runs longer than L (at most one) and the tail
+		for (size_t j = k[r].c0 + L; j < k[r].c1; j++)
+			a[r] = ck_step(a[r], _mm_loadu_si128(reinterpret_cast<const __m128i*>(p + 8 * j)));
+		if (k[r].tail) {
+			uint16_t tail[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+			std::memcpy(tail, p + 8 * k[r].c1, (n - 8 * k[r].c1) * 2);
+			a[r] = ck_step(a[r], _mm_loadu_si128(reinterpret_cast<const __m128i*>(tail)));
+		}
+	}
+	return ck_join(a, R, n);
+}
+
+// Slices of this many bytes and more are copied and checksummed on the host
+// copy threads.
+static constexpr size_t kParallelBytes = 256u << 10;
+
+static void copy_parallel(void* dst, const void* src, size_t bytes) {
+	const size_t parts = 8;
+	const size_t step = ((bytes + parts - 1) / parts + 15) & ~(size_t)15;
+	parallel_parts(parts, [&](size_t q) {
+		const size_t b0 = std::min(bytes, q * step), b1 = std::min(bytes, b0 + step);
+		if (b1 > b0) std::memcpy(static_cast<char*>(dst) + b0, static_cast<const char*>(src) + b0, b1 - b0);
+	});
+}
+
+static uint16_t checksum_parallel(const uint16_t* p, size_t n) {
+	constexpr size_t parts = 8;
+	__m128i v[parts];
+	parallel_parts(parts, [&](size_t r) { v[r] = ck_run_value(p, n, ck_run(n, parts, r)); });
+	return ck_join(v, parts, n);
+}
+
+// Stage a large slice: copy and checksum in parallel on the host copy threads
+// (the same bytes and checksum as memcpy + slice_checksum).
+static uint16_t stage_large(uint16_t* slot, const void* src, size_t size, size_t n,
+                            size_t slot_bytes) {
+	copy_parallel(slot, src, size);
+	std::memset(reinterpret_cast<char*>(slot) + size, 0, slot_bytes - size);
+	return checksum_parallel(slot, n);
 }
 
 static std::future<void> ready_future() {
@@ -127,8 +197,13 @@ bool PAR2ProcHIP::allocOutputs() {
 		hipFree(dout);
 		dout = nullptr;
 	}
+	if (hout) {
+		hipHostFree(hout);
+		hout = nullptr;
+	}
 	if (!allocWords || outputExponents.empty()) return true;
 	hipError_t e = hipMalloc((void**)&dout, outputExponents.size() * strideWords * sizeof(uint16_t));
+	if (e == hipSuccess) e = hipHostMalloc((void**)&hout, strideWords * sizeof(uint16_t), 0);
 	if (e != hipSuccess) {
 		setErr("output allocation", e);
 		return false;
@@ -211,9 +286,13 @@ std::future<void> PAR2ProcHIP::addInputT(const void* buffer, size_t size, T numO
 	uint16_t* slot = st.hsrc + (size_t)currentStagingInputs * strideWords;
 	const size_t payloadWords = words - 1;
 	if (size > payloadWords * 2) size = payloadWords * 2;
-	std::memcpy(slot, buffer, size);
-	std::memset(reinterpret_cast<char*>(slot) + size, 0, (words * 2) - size);
-	slot[payloadWords] = slice_checksum(slot, payloadWords);
+	if (size >= kParallelBytes) {
+		slot[payloadWords] = stage_large(slot, buffer, size, payloadWords, words * 2);
+	} else {
+		std::memcpy(slot, buffer, size);
+		std::memset(reinterpret_cast<char*>(slot) + size, 0, (words * 2) - size);
+		slot[payloadWords] = slice_checksum(slot, payloadWords);
+	}
 	setCoeffs(st, currentStagingInputs, numOrCoeffs);
 	currentStagingInputs++;
 	if (currentStagingInputs >= inputBatchSize || flushNow) flush();
@@ -323,16 +402,24 @@ std::future<bool> PAR2ProcHIP::getOutput(unsigned index, void* output) {
 		pr.set_value(true);
 		return pr.get_future();
 	}
-	std::vector<uint16_t> tmp(words);
-	hipError_t e = hipMemcpy(tmp.data(), dout + (size_t)index * strideWords, words * 2,
+	// through a pinned buffer (DMA at full rate), checksum verified there
+	hipError_t e = hipMemcpy(hout, dout + (size_t)index * strideWords, words * 2,
 	                         hipMemcpyDeviceToHost);
 	if (e != hipSuccess) {
 		setErr("output copy", e);
 		pr.set_value(false);
 		return pr.get_future();
 	}
-	std::memcpy(output, tmp.data(), currentSliceSize);
-	pr.set_value(slice_checksum(tmp.data(), words - 1) == tmp[words - 1]);
+	const size_t n = words - 1;
+	bool valid;
+	if (currentSliceSize >= kParallelBytes) {
+		copy_parallel(output, hout, currentSliceSize);
+		valid = checksum_parallel(hout, n) == hout[n];
+	} else {
+		std::memcpy(output, hout, currentSliceSize);
+		valid = slice_checksum(hout, n) == hout[n];
+	}
+	pr.set_value(valid);
 	return pr.get_future();
 }
 
@@ -341,6 +428,8 @@ void PAR2ProcHIP::freeProcessingMem() {
 	hipStreamSynchronize(strm);
 	if (dout) hipFree(dout);
 	dout = nullptr;
+	if (hout) hipHostFree(hout);
+	hout = nullptr;
 	allocOutputsCount = 0;
 }
 

@@ -80,6 +80,7 @@ private:
 	std::vector<Staging> staging;
 	std::vector<uint16_t> outputExponents;
 	uint16_t* dout = nullptr;     // numOutputs x strideWords
+	uint16_t* hout = nullptr;     // pinned: one output slice on its way back (getOutput)
 	size_t sliceSize = 0, currentSliceSize = 0;
 	size_t allocWords = 0;        // per-slice words allocated (payload + checksum)
 	size_t words = 0;             // current payload words + 1 checksum word
