@@ -148,6 +148,7 @@ static int run_single(const BatchJob& jb, int cfg, BatchJob* djob, BatchTile* dt
 		                            atomic ? 1 : 0, 0, blocks, s);
 		hipEventRecord(b, s);
 		if (e != hipSuccess) {
+			(void)hipStreamSynchronize(s);  // the tile copy above reads `tiles`
 			hipEventDestroy(a);
 			hipEventDestroy(b);
 			return hip_fail(e, "autotune launch");

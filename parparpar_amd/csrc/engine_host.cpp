@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "../../include/parhip.h"
@@ -68,6 +69,17 @@ static int sync(hipStream_t s) {
 
 // Generic host-pointer contraction: G input blocks (pointers), K output blocks
 // (pointers), coefficients K x G (ld = coeff_ld), len words each.
+// Run `body` (which enqueues on s); on failure, drain s before returning so
+// no copy still in flight reads host memory the caller's frame is releasing.
+static int drained(hipStream_t s, const std::function<int()>& body) {
+	const int rc = body();
+	if (rc) {
+		(void)hipStreamSynchronize(s);
+		return rc;
+	}
+	return sync(s);
+}
+
 static int host_contract(uint64_t* const* outs, size_t K, const uint64_t* const* ins, size_t G,
                          const uint64_t* coeff, size_t coeff_ld, size_t len, int accumulate) {
 	if (K == 0 || G == 0 || len == 0) return PARHIP_OK;
@@ -77,31 +89,33 @@ static int host_contract(uint64_t* const* outs, size_t K, const uint64_t* const*
 	RC(words_bytes(K, G, &cb));
 	hipStream_t s;
 	RC(host_stream(&s));
-	DevBuf din, dout, dco;
-	RC(din.alloc(in_bytes, s));
-	RC(dout.alloc(out_bytes, s));
-	RC(dco.alloc(cb, s));
-	for (size_t g = 0; g < G; g++) RC(h2d(din.as<uint64_t>() + g * len, ins[g], blk, s));
-	if (accumulate)
-		for (size_t k = 0; k < K; k++) RC(h2d(dout.as<uint64_t>() + k * len, outs[k], blk, s));
 	std::vector<uint64_t> c(K * G);
-	for (size_t k = 0; k < K; k++)
-		for (size_t g = 0; g < G; g++) c[k * G + g] = coeff[k * coeff_ld + g];
-	RC(h2d(dco.p, c.data(), cb, s));
-	MacArgs a{};
-	a.in_base = din.as<uint64_t>();
-	a.in_stride = len;
-	a.out_base = dout.as<uint64_t>();
-	a.out_stride = len;
-	a.coeff = dco.as<uint64_t>();
-	a.coeff_ld = G;
-	a.N = G;
-	a.R = K;
-	a.W = len;
-	a.accumulate = accumulate;
-	RC(run_contraction(a, s));
-	for (size_t k = 0; k < K; k++) RC(d2h(outs[k], dout.as<uint64_t>() + k * len, blk, s));
-	return sync(s);
+	DevBuf din, dout, dco;
+	return drained(s, [&]() -> int {
+		RC(din.alloc(in_bytes, s));
+		RC(dout.alloc(out_bytes, s));
+		RC(dco.alloc(cb, s));
+		for (size_t g = 0; g < G; g++) RC(h2d(din.as<uint64_t>() + g * len, ins[g], blk, s));
+		if (accumulate)
+			for (size_t k = 0; k < K; k++) RC(h2d(dout.as<uint64_t>() + k * len, outs[k], blk, s));
+		for (size_t k = 0; k < K; k++)
+			for (size_t g = 0; g < G; g++) c[k * G + g] = coeff[k * coeff_ld + g];
+		RC(h2d(dco.p, c.data(), cb, s));
+		MacArgs a{};
+		a.in_base = din.as<uint64_t>();
+		a.in_stride = len;
+		a.out_base = dout.as<uint64_t>();
+		a.out_stride = len;
+		a.coeff = dco.as<uint64_t>();
+		a.coeff_ld = G;
+		a.N = G;
+		a.R = K;
+		a.W = len;
+		a.accumulate = accumulate;
+		RC(run_contraction(a, s));
+		for (size_t k = 0; k < K; k++) RC(d2h(outs[k], dout.as<uint64_t>() + k * len, blk, s));
+		return PARHIP_OK;
+	});
 }
 
 // Contraction of N contiguous host input slices (W words each) against a
@@ -394,16 +408,18 @@ int parhip_gf64_solve_and_reconstruct(uint64_t* A, uint64_t* rhsBlocks, size_t n
 		hipStream_t s;
 		RC(host_stream(&s));
 		DevBuf drhs, dout, dE;
-		RC(drhs.alloc(rb, s));
-		RC(dout.alloc(rb, s));
-		RC(dE.alloc(eb, s));
-		RC(h2d(drhs.p, rhsBlocks, rb, s));
-		RC(h2d(dE.p, E.data(), eb, s));
-		RC(parhip_d_gf64_recovery_with_coeff(drhs.as<uint64_t>(), n, blockSize64,
-		                                     dout.as<uint64_t>(), n, blockSize64, blockSize64,
-		                                     dE.as<uint64_t>(), 0, s));
-		RC(d2h(rhsBlocks, dout.p, rb, s));
-		RC(sync(s));
+		RC(drained(s, [&]() -> int {
+			RC(drhs.alloc(rb, s));
+			RC(dout.alloc(rb, s));
+			RC(dE.alloc(eb, s));
+			RC(h2d(drhs.p, rhsBlocks, rb, s));
+			RC(h2d(dE.p, E.data(), eb, s));
+			RC(parhip_d_gf64_recovery_with_coeff(drhs.as<uint64_t>(), n, blockSize64,
+			                                     dout.as<uint64_t>(), n, blockSize64, blockSize64,
+			                                     dE.as<uint64_t>(), 0, s));
+			RC(d2h(rhsBlocks, dout.p, rb, s));
+			return PARHIP_OK;
+		}));
 	}
 	if (singular) return fail(PARHIP_ERR_SINGULAR, "solve_and_reconstruct: singular matrix");
 	return PARHIP_OK;
@@ -444,16 +460,18 @@ int parhip_gf64_region_mul_arr(uint64_t* out, const uint64_t* in, const uint64_t
 	hipStream_t s;
 	RC(host_stream(&s));
 	DevBuf din, dout, dco;
-	RC(din.alloc(bytes, s));
-	RC(dout.alloc(bytes, s));
-	RC(dco.alloc(cb, s));
-	RC(h2d(din.p, in, bytes, s));
-	RC(h2d(dco.p, coeff, cb, s));
-	HIPC(launch_region_mul_arr(dout.as<uint64_t>(), din.as<uint64_t>(), dco.as<uint64_t>(), len,
-	                           n_coeff, s),
-	     "k_region_mul_arr");
-	RC(d2h(out, dout.p, bytes, s));
-	return sync(s);
+	return drained(s, [&]() -> int {
+		RC(din.alloc(bytes, s));
+		RC(dout.alloc(bytes, s));
+		RC(dco.alloc(cb, s));
+		RC(h2d(din.p, in, bytes, s));
+		RC(h2d(dco.p, coeff, cb, s));
+		HIPC(launch_region_mul_arr(dout.as<uint64_t>(), din.as<uint64_t>(), dco.as<uint64_t>(), len,
+		                           n_coeff, s),
+		     "k_region_mul_arr");
+		RC(d2h(out, dout.p, bytes, s));
+		return PARHIP_OK;
+	});
 }
 
 int parhip_gf64_region_muladd_arr(uint64_t* out, const uint64_t* in, const uint64_t* coeff,

@@ -54,6 +54,8 @@ int parhip_d_gf16_rec_matrix_compute(const uint8_t* inputValid, uint32_t numInpu
 	hipError_t e = hipMemcpyAsync(d_vidx, vidx.data(), vidx.size() * 2, hipMemcpyHostToDevice, s);
 	if (e == hipSuccess)
 		e = hipMemcpyAsync(d_midx, midx.data(), midx.size() * 2, hipMemcpyHostToDevice, s);
+	// vidx / midx are this frame's: let the copies finish before any return
+	if (e == hipSuccess) e = hipStreamSynchronize(s);
 	if (e != hipSuccess) return hip_fail(e, "hipMemcpy H2D");
 	uint16_t* inv = nullptr;
 	for (;;) {  // invert_loop: a zero pivot drops that exponent and retries
