@@ -11,6 +11,14 @@
 // [Cauchy | identity], A^-1 comes from the closed form of a Cauchy inverse
 // (repair_kernels.hip), and X is a second contraction.
 //
+// The closed form holds only for a true Cauchy matrix.  When a missing input's
+// point equals a used recovery point (overlapping ranges, which the reference
+// accepts: test/par3-fenger-padded-engine.js:110-112), the reference's
+// coefficient for that pair is 1 (src/par3_engine.cc:392-394) and A is no
+// longer Cauchy; then A is inverted on the host by Gauss-Jordan with the
+// reference's pivot rule (src/par3_engine.cc:989-1045), and a singular A gives
+// PARHIP_ERR_SINGULAR (-1), the reference's result.
+//
 // The server keeps archives resident in HBM, admits requests in arrival
 // order while the number in flight and their device memory stay within the
 // configured limits, gives every admitted request its own stream (their work
@@ -31,6 +39,7 @@
 #include "../../include/parhip.h"
 #include "../../include/parhip_repair.h"
 #include "engine_internal.h"
+#include "gf64_host.h"
 #include "kernels.h"
 
 namespace parhip {
@@ -47,6 +56,7 @@ struct RepairArchive {
 struct RepairPlan {
 	std::vector<uint64_t> host_idx;  // [xs_known | xs_missing | ys]
 	std::vector<const uint64_t*> in_ptrs;
+	std::vector<uint64_t> host_inv;  // A^-1 from the host (colliding points only)
 	void* ws = nullptr;
 	size_t ws_bytes = 0;
 };
@@ -110,6 +120,27 @@ static int repair_enqueue(const RepairArchive& ar, const uint64_t* missing, uint
 		ys[e] = ar.fr + r;
 		plan.in_ptrs[nk + e] = ar.rec + r * ar.rec_stride;
 	}
+	// A point shared by a missing input and a used recovery row makes A a
+	// Cauchy matrix with some entries replaced by 1: invert it on the host.
+	bool collide = false;
+	{
+		std::vector<uint64_t> ys_sorted(ys, ys + k);
+		std::sort(ys_sorted.begin(), ys_sorted.end());
+		for (uint64_t m = 0; m < k && !collide; m++)
+			collide = std::binary_search(ys_sorted.begin(), ys_sorted.end(), xs_miss[m]);
+	}
+	if (collide) {
+		std::vector<uint64_t> A(k * k);
+		for (uint64_t eq = 0; eq < k; eq++)
+			for (uint64_t m = 0; m < k; m++) {
+				const uint64_t d = ys[eq] ^ xs_miss[m];
+				A[eq * k + m] = d ? host_gf64_inv(d) : 1u;
+			}
+		plan.host_inv.resize(k * k);
+		if (host_gauss_jordan(A.data(), plan.host_inv.data(), k))
+			return fail(PARHIP_ERR_SINGULAR, "repair: the %llu x %llu system is singular",
+			            (unsigned long long)k, (unsigned long long)k);
+	}
 	// workspace carve-up
 	plan.ws_bytes = repair_ws_bytes(N, W, k);
 	int rc = dev_alloc(&plan.ws, plan.ws_bytes, s);
@@ -128,7 +159,12 @@ static int repair_enqueue(const RepairArchive& ar, const uint64_t* missing, uint
 	if (e == hipSuccess)
 		e = hipMemcpyAsync(d_ptrs, plan.in_ptrs.data(), N * 8, hipMemcpyHostToDevice, s);
 	if (e == hipSuccess) e = launch_repair_coeffs(d_coef, N, d_idx, nk, d_ys, k, s);
-	if (e == hipSuccess) e = launch_cauchy_inverse(d_ys, d_idx + nk, k, d_alpha, d_beta, d_inv, s);
+	if (e == hipSuccess) {
+		if (collide)
+			e = hipMemcpyAsync(d_inv, plan.host_inv.data(), k * k * 8, hipMemcpyHostToDevice, s);
+		else
+			e = launch_cauchy_inverse(d_ys, d_idx + nk, k, d_alpha, d_beta, d_inv, s);
+	}
 	if (e != hipSuccess) return hip_fail(e, "repair setup");
 	// RHS = [C_known | I] * [known slices ; recovery rows]
 	MacArgs a{};

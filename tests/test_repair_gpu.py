@@ -121,3 +121,106 @@ def test_server_budget_admission(gpu):
     st = srv.stats()
     assert st["peak_inflight"] == 1 and st["peak_bytes"] <= int(one * 1.5)
     srv.close()
+
+
+# Overlapping input and recovery ranges, the reference's "collision fallback"
+# shape (test/par3-fenger-padded-engine.js:110-112): inputs at 0x10000 + i,
+# recovery points 66300 + r.  Missing input 764 + r shares its point with
+# recovery row r, so A holds the reference's 1 (src/par3_engine.cc:392-394)
+# where a Cauchy entry would divide by zero.
+COL_N, COL_R, COL_FI, COL_FR = 1000, 512, 0x10000, 66300
+
+
+def _oracle_repair(oracle, D, REC, fi, fr, missing, rows):
+    n, r = D.shape[0], REC.shape[0]
+    co = oracle.cauchy(n, r, fi, fr)
+    known = [i for i in range(n) if i not in set(missing)]
+    A = np.ascontiguousarray(co[rows][:, missing])
+    rhs = np.ascontiguousarray(REC[rows]).copy()
+    rhs ^= oracle.recovery_with_coeff(np.ascontiguousarray(D[known]),
+                                      np.ascontiguousarray(co[rows][:, known]))
+    return oracle.solve_and_reconstruct(A, rhs)
+
+
+@pytest.mark.parametrize("missing,rows", [
+    ([764, 765, 800, 3, 999], None),          # two points shared with rows 0 and 1
+    ([0, 2, 770], [6, 1, 2]),                 # a shared point on a chosen row
+    ([764 + e for e in range(24)] + [5, 17], None),  # shared points on most rows
+])
+def test_repair_colliding_points(gpu, oracle, missing, rows):
+    import torch
+    from parparpar_amd import device, repair
+    g = torch.Generator(device="cuda")
+    g.manual_seed(77)
+    data = torch.randint(-2 ** 63, 2 ** 63 - 1, (COL_N, 16), dtype=torch.int64, device="cuda",
+                         generator=g)
+    rec = device.recovery(data, COL_R, COL_FI, COL_FR)
+    torch.cuda.synchronize()
+    k = len(missing)
+    use_rows = list(range(k)) if rows is None else rows
+    D = data.cpu().numpy().view(np.uint64)
+    REC = rec.cpu().numpy().view(np.uint64)
+    rc, _, sol = _oracle_repair(oracle, D, REC, COL_FI, COL_FR, missing, use_rows)
+    assert rc == 0
+    assert np.array_equal(sol, D[missing])
+    out = repair.repair(data, rec, missing, first_input=COL_FI, first_recovery=COL_FR, rows=rows)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint64), sol)
+
+
+def test_repair_singular_returns_minus_one(gpu, oracle):
+    """Missing inputs 764 and 765 on rows 0 and 1: A = [[1, 1], [1, 1]] (their
+    points differ by 1), singular.  The reference's solve returns -1."""
+    import torch
+    from parparpar_amd import ParhipError, device, repair
+    data = torch.arange(COL_N * 4, dtype=torch.int64, device="cuda").reshape(COL_N, 4)
+    rec = device.recovery(data, COL_R, COL_FI, COL_FR)
+    D = data.cpu().numpy().view(np.uint64)
+    REC = rec.cpu().numpy().view(np.uint64)
+    rc, _, _ = _oracle_repair(oracle, D, REC, COL_FI, COL_FR, [764, 765], [0, 1])
+    assert rc == -1
+    with pytest.raises(ParhipError) as ei:
+        repair.repair(data, rec, [764, 765], first_input=COL_FI, first_recovery=COL_FR)
+    assert ei.value.code == -1
+
+
+def test_server_colliding_points(gpu):
+    """The server gives the same bytes alone and in a mix for colliding points,
+    and reports -1 for the singular request without disturbing the others."""
+    import torch
+    from parparpar_amd import ParhipError, device
+    from parparpar_amd.repair import RepairServer, copy_result
+    g = torch.Generator(device="cuda")
+    g.manual_seed(78)
+    data = torch.randint(-2 ** 63, 2 ** 63 - 1, (COL_N, 64), dtype=torch.int64, device="cuda",
+                         generator=g)
+    rec = device.recovery(data, COL_R, COL_FI, COL_FR)
+    torch.cuda.synchronize()
+    srv = RepairServer(4 << 30, max_inflight=4)
+    srv.add_archive(data, rec, first_input=COL_FI, first_recovery=COL_FR)
+    reqs = [[764, 765, 800, 3, 999], [766, 770, 771], [764, 765], [10, 767, 999, 500]]
+    singular = 2
+    alone = []
+    for i, miss in enumerate(reqs):
+        rid = srv.submit(0, miss)
+        if i == singular:
+            with pytest.raises(ParhipError) as ei:
+                srv.wait(rid)
+            assert ei.value.code == -1
+            alone.append(None)
+        else:
+            srv.wait(rid)
+            alone.append(copy_result(srv, rid, 64))
+        srv.release(rid)
+    ids = [srv.submit(0, miss) for miss in reqs]
+    for i, (rid, miss) in enumerate(zip(ids, reqs)):
+        if i == singular:
+            with pytest.raises(ParhipError):
+                srv.wait(rid)
+        else:
+            srv.wait(rid)
+            got = copy_result(srv, rid, 64)
+            assert torch.equal(got, alone[i])
+            assert torch.equal(got, data[torch.as_tensor(miss, device="cuda")])
+        srv.release(rid)
+    srv.close()
