@@ -47,6 +47,9 @@ extern "C" {
 #define PARHIP_METHOD_MFMA 2     /* GF(2) bit-matrix on fp4 MFMA (exact 0/1 counts) */
 #define PARHIP_METHOD_MFMA_I8 3  /* GF(2) bit-matrix on int8 MFMA (int32 parity)    */
 #define PARHIP_METHOD_TABLE8 4   /* 8-bit split tables in LDS (256-entry rows)      */
+#define PARHIP_METHOD_DYADIC 5   /* plain Cauchy calls: 8 x 8 XOR blocks as subset  */
+                                 /* convolutions (27 products per block), nibble    */
+                                 /* tables in LDS; other calls as AUTO              */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int parhip_version(void);

@@ -126,6 +126,7 @@ METHOD_TABLE = 1
 METHOD_MFMA = 2
 METHOD_MFMA_I8 = 3
 METHOD_TABLE8 = 4
+METHOD_DYADIC = 5
 
 
 class ParhipError(RuntimeError):

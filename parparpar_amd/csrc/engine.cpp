@@ -125,14 +125,17 @@ void dev_free(void* p, hipStream_t s) {
 // are safe.  Entries are built on the device (k_cauchy); a user waits on the
 // entry's `built` event, and an entry is only evicted when no caller holds it
 // (reference count) and after the device has drained.
+// kind 0: the R x N matrix (k_cauchy); kind 1: the XOR-block coefficients
+// of the dyadic contraction (k_dyadic_coeffs, Q x blocks x 8 words).
 struct CoeffKey {
 	int dev;
+	int kind;
 	uint64_t n, r, fi, fr;
 	bool operator<(const CoeffKey& o) const {
-		return std::tie(dev, n, r, fi, fr) < std::tie(o.dev, o.n, o.r, o.fi, o.fr);
+		return std::tie(dev, kind, n, r, fi, fr) < std::tie(o.dev, o.kind, o.n, o.r, o.fi, o.fr);
 	}
 	bool operator==(const CoeffKey& o) const {
-		return std::tie(dev, n, r, fi, fr) == std::tie(o.dev, o.n, o.r, o.fi, o.fr);
+		return std::tie(dev, kind, n, r, fi, fr) == std::tie(o.dev, o.kind, o.n, o.r, o.fi, o.fr);
 	}
 };
 struct CoeffEntry {
@@ -145,9 +148,23 @@ static std::map<CoeffKey, CoeffEntry> g_coeff;
 static std::list<CoeffKey> g_coeff_lru;
 static const size_t kCoeffCacheMax = 16;
 
+// Words of the dyadic coefficients for (N, R, fi, fr).
+static uint64_t dyadic_coeff_words(uint64_t n, uint64_t r, uint64_t fi, uint64_t fr) {
+	const uint64_t nblocks = ((fi & 7) + n + 7) / 8, Q = ((fr & 7) + r + 7) / 8;
+	return nblocks * Q * 8;
+}
+
+static int coeff_acquire_kind(int kind, uint64_t n, uint64_t r, uint64_t fi, uint64_t fr,
+                              hipStream_t s, const uint64_t** out);
+
 int coeff_acquire(uint64_t n, uint64_t r, uint64_t fi, uint64_t fr, hipStream_t s,
                   const uint64_t** out) {
-	CoeffKey key{t_device, n, r, fi, fr};
+	return coeff_acquire_kind(0, n, r, fi, fr, s, out);
+}
+
+static int coeff_acquire_kind(int kind, uint64_t n, uint64_t r, uint64_t fi, uint64_t fr,
+                              hipStream_t s, const uint64_t** out) {
+	CoeffKey key{t_device, kind, n, r, fi, fr};
 	std::unique_lock<std::mutex> lk(g_coeff_mu);
 	auto it = g_coeff.find(key);
 	if (it != g_coeff.end()) {
@@ -176,15 +193,20 @@ int coeff_acquire(uint64_t n, uint64_t r, uint64_t fi, uint64_t fr, hipStream_t 
 		if (!evicted) break;
 	}
 	CoeffEntry ent;
-	hipError_t e = hipMalloc((void**)&ent.d, n * r * sizeof(uint64_t));
+	const uint64_t words = kind ? dyadic_coeff_words(n, r, fi, fr) : n * r;
+	hipError_t e = hipMalloc((void**)&ent.d, words * sizeof(uint64_t));
 	if (e != hipSuccess) return hip_fail(e, "hipMalloc(coefficients)");
-	{
+	if (kind) {
+		ProfScope ps("dyadic_coeffs", s);
+		e = launch_dyadic_coeffs(ent.d, ((fi & 7) + n + 7) / 8, ((fr & 7) + r + 7) / 8, fi & ~7ull,
+		                         fr & ~7ull, s);
+	} else {
 		ProfScope ps("cauchy", s);
 		e = launch_cauchy(ent.d, n, r, fi, fr, s);
 	}
 	if (e != hipSuccess) {
 		hipFree(ent.d);
-		return hip_fail(e, "k_cauchy");
+		return hip_fail(e, kind ? "k_dyadic_coeffs" : "k_cauchy");
 	}
 	hipEventCreateWithFlags(&ent.built, hipEventDisableTiming);
 	hipEventRecord(ent.built, s);
@@ -508,6 +530,120 @@ int run_contraction_tables(MacArgs a, hipStream_t s) {
 	return PARHIP_OK;
 }
 
+// ------------------------------------------------------------------ dyadic plan
+// Launch shape of the XOR-block contraction: 4 words per thread for slices of
+// at least 1024 words (tables amortised over 1024 words), else 2; split-K rows
+// for two rounds of work-groups per resident slot (2 per CU: 251 / 157 VGPRs).
+DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev) {
+	DyadicPlan p;
+	p.pre_in = fi & 7;
+	p.pre_out = fr & 7;
+	p.nblocks = (p.pre_in + N + 7) / 8;
+	p.Q = (p.pre_out + R + 7) / 8;
+	p.wpt = W >= 1024 ? 4 : 2;
+	p.ib = 1;
+	if (t_tune_wpt || t_tune_ib) {
+		const int wpt = t_tune_wpt ? t_tune_wpt : p.wpt, ib = t_tune_ib ? t_tune_ib : p.ib;
+		if (mac_dyadic_supported(wpt, ib)) {
+			p.wpt = wpt;
+			p.ib = ib;
+		}
+	}
+	const uint64_t tiles = (W + 256ull * p.wpt - 1) / (256ull * p.wpt);
+	const uint64_t base = tiles * p.Q;
+	const uint64_t target = (uint64_t)device_cus(dev) * 2 * 2;
+	uint64_t groups = base < target ? (target + base - 1) / base : 1;
+	if (t_tune_groups > 0) groups = (uint64_t)t_tune_groups;
+	// at least 2 batches of blocks per row, so the table builds stay amortised
+	groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, p.nblocks / (2 * p.ib)));
+	groups = std::min<uint64_t>(groups, 65535);
+	uint64_t per = (p.nblocks + groups - 1) / groups;
+	per = (per + p.ib - 1) / p.ib * p.ib;
+	p.groups = (p.nblocks + per - 1) / per;
+	p.blocks_per_group = per;
+	p.part_bytes = p.groups > 1 ? (size_t)xor_reduce_scratch_words(p.groups, R, W) * sizeof(uint64_t) : 0;
+	return p;
+}
+
+bool dyadic_applies(const MacArgs& a) {
+	if (t_method != PARHIP_METHOD_AUTO && t_method != PARHIP_METHOD_DYADIC) return false;
+	// 27 products per 8 x 8 block against 8 per input and output: the direct
+	// tables cost less below 4 outputs
+	if (a.in_ptrs || a.out_ptrs || a.R < 4) return false;
+	const uint64_t Q = ((a.R + 7) / 8) + 1;
+	return Q <= 65535;
+}
+
+hipError_t enqueue_dyadic(const MacArgs& a, const DyadicPlan& p, const uint64_t* gp,
+                          uint64_t gp_ld, void* ws, hipStream_t s) {
+	DyadicArgs d{};
+	d.in_base = a.in_base;
+	d.in_stride = a.in_stride;
+	d.out_base = a.out_base;
+	d.out_stride = a.out_stride;
+	d.gp = gp;
+	d.gp_ld = gp_ld;
+	d.N = a.N;
+	d.R = a.R;
+	d.W = a.W;
+	d.pre_in = p.pre_in;
+	d.pre_out = p.pre_out;
+	d.nblocks = p.nblocks;
+	d.blocks_per_group = p.blocks_per_group;
+	d.partial = p.groups > 1 ? (uint64_t*)ws : nullptr;
+	d.accumulate = a.accumulate;
+	hipError_t e;
+	{
+		ProfScope ps("mac_dyadic", s);
+		e = launch_mac_dyadic(d, p.groups, p.wpt, p.ib, s);
+	}
+	if (e == hipSuccess && p.groups > 1) {
+		ProfScope ps("xor_reduce", s);
+		e = launch_xor_reduce(d.partial, p.groups, a.R, a.W, a.out_base, a.out_stride, nullptr,
+		                      a.accumulate, s);
+	}
+	return e;
+}
+
+// The plain Cauchy contraction of a call with n_total inputs at points
+// (fi + i, fr + r), over its inputs [first, first + a.N) (a chunk of a streamed
+// call; first = 0 and n_total = a.N for a whole call): the XOR-block kernel
+// where it applies, else the cached R x n_total matrix and run_contraction.
+// Either way one cached coefficient set serves every chunk of the call.
+int run_cauchy_contraction(MacArgs a, uint64_t fi, uint64_t fr, hipStream_t s, uint64_t first,
+                           uint64_t n_total) {
+	if (a.N == 0 || a.R == 0 || a.W == 0) return PARHIP_OK;
+	if (n_total == 0) n_total = a.N;
+	if (dyadic_applies(a)) {
+		// the chunk's own virtual blocks start at block b0 of the call's
+		const uint64_t cfi = fi + first;
+		const uint64_t b0 = ((cfi & ~7ull) - (fi & ~7ull)) / 8;
+		const DyadicPlan p = plan_dyadic(a.N, a.R, a.W, cfi, fr, t_device);
+		const uint64_t* gp = nullptr;
+		int rc = coeff_acquire_kind(1, n_total, a.R, fi, fr, s, &gp);
+		if (rc) return rc;
+		void* ws = nullptr;
+		rc = dev_alloc(&ws, p.part_bytes, s);
+		if (rc) {
+			coeff_release(gp);
+			return rc;
+		}
+		const uint64_t call_blocks = ((fi & 7) + n_total + 7) / 8;
+		const hipError_t e = enqueue_dyadic(a, p, gp + b0 * 8, call_blocks, ws, s);
+		dev_free(ws, s);
+		coeff_release(gp);
+		return e == hipSuccess ? PARHIP_OK : hip_fail(e, "k_mac_dyadic");
+	}
+	const uint64_t* coeff = nullptr;
+	int rc = coeff_acquire(n_total, a.R, fi, fr, s, &coeff);
+	if (rc) return rc;
+	a.coeff = coeff + first;
+	a.coeff_ld = n_total;
+	rc = run_contraction(a, s);
+	coeff_release(coeff);
+	return rc;
+}
+
 }  // namespace parhip
 
 using namespace parhip;
@@ -528,7 +664,7 @@ int parhip_set_device(int device) {
 }
 
 int parhip_set_method(int method) {
-	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_TABLE8)
+	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_DYADIC)
 		return fail(PARHIP_ERR_ARG, "parhip_set_method: unknown method %d", method);
 	t_method = method;
 	return PARHIP_OK;
@@ -626,15 +762,20 @@ int parhip_d_gf64_recovery(const uint64_t* d_inputs, size_t numInputs, size_t in
                            size_t blockSize64, uint64_t firstInput, uint64_t firstRecovery,
                            int accumulate, void* stream) {
 	if (numInputs == 0 || numRecovery == 0) return PARHIP_OK;
-	hipStream_t s = (hipStream_t)stream;
-	const uint64_t* coeff = nullptr;
-	int rc = coeff_acquire(numInputs, numRecovery, firstInput, firstRecovery, s, &coeff);
-	if (rc) return rc;
-	rc = parhip_d_gf64_recovery_with_coeff(d_inputs, numInputs, inputStride64, d_recovery,
-	                                       numRecovery, recoveryStride64, blockSize64, coeff,
-	                                       accumulate, stream);
-	coeff_release(coeff);
-	return rc;
+	if (!d_inputs || !d_recovery) return fail(PARHIP_ERR_ARG, "recovery: null pointer");
+	if (inputStride64 < blockSize64 || recoveryStride64 < blockSize64)
+		return fail(PARHIP_ERR_ARG, "recovery: stride < blockSize64");
+	device_cus(t_device);
+	MacArgs a{};
+	a.in_base = d_inputs;
+	a.in_stride = inputStride64;
+	a.out_base = d_recovery;
+	a.out_stride = recoveryStride64;
+	a.N = numInputs;
+	a.R = numRecovery;
+	a.W = blockSize64;
+	a.accumulate = accumulate ? 1 : 0;
+	return run_cauchy_contraction(a, firstInput, firstRecovery, (hipStream_t)stream, 0, numInputs);
 }
 
 int parhip_d_gf64_recovery_ptrs(const uint64_t* const* d_in_ptrs, size_t numIn,

@@ -143,6 +143,27 @@ static int stream_contract(const FillFn& fill, uint64_t N, uint64_t W, uint64_t*
 	return stream_slices(N, (size_t)W * sizeof(uint64_t), fill, consume, s);
 }
 
+// The plain Cauchy contraction (points fi + i, fr + r) of N host slices,
+// streamed like stream_contract: chunk [first, first + count) runs as that part
+// of the whole call, on one cached coefficient set.
+static int stream_cauchy(const FillFn& fill, uint64_t N, uint64_t W, uint64_t* dout, uint64_t R,
+                         uint64_t fi, uint64_t fr, int accumulate, hipStream_t s) {
+	const ConsumeFn consume = [&](const void* src, uint64_t first, uint64_t count,
+	                              hipStream_t st) {
+		MacArgs a{};
+		a.in_base = (const uint64_t*)src;
+		a.in_stride = W;
+		a.out_base = dout;
+		a.out_stride = W;
+		a.N = count;
+		a.R = R;
+		a.W = W;
+		a.accumulate = (accumulate || first > 0) ? 1 : 0;
+		return run_cauchy_contraction(a, fi, fr, st, first, N);
+	};
+	return stream_slices(N, (size_t)W * sizeof(uint64_t), fill, consume, s);
+}
+
 // memcpy split into 4 MiB parts over the copy threads.
 static void parallel_copy(void* dst, const void* src, size_t bytes) {
 	const size_t part = 4u << 20;
@@ -243,17 +264,11 @@ int parhip_gf64_compute_recovery_blocks(const uint64_t* inputs, size_t numInputs
 	hipStream_t s;
 	RC(host_stream(&s));
 	if (use_streaming(in_bytes)) {
-		size_t cb;
-		RC(words_bytes(numRecovery, numInputs, &cb));
-		DevBuf dout, dco;
+		DevBuf dout;
 		RC(dout.alloc(out_bytes, s));
-		RC(dco.alloc(cb, s));
-		RC(parhip_d_gf64_build_cauchy(dco.as<uint64_t>(), numInputs, numRecovery, firstInput,
-		                              firstRecovery, s));
 		if (accumulate) RC(h2d(dout.p, recovery, out_bytes, s));
-		RC(stream_contract(memcpy_fill(inputs, blockSize64 * 8), numInputs, blockSize64,
-		                   dout.as<uint64_t>(), numRecovery, dco.as<uint64_t>(), numInputs, accumulate,
-		                   s));
+		RC(stream_cauchy(memcpy_fill(inputs, blockSize64 * 8), numInputs, blockSize64,
+		                 dout.as<uint64_t>(), numRecovery, firstInput, firstRecovery, accumulate, s));
 		RC(d2h(recovery, dout.p, out_bytes, s));
 		return sync(s);
 	}
@@ -312,30 +327,26 @@ int parhip_gf64_compute_recovery_batches(const uint64_t* const* batches, const s
 		return fail(PARHIP_ERR_RANGE, "compute_recovery_batches: batches hold %llu slices, numInputs is %zu",
 		            (unsigned long long)starts[numBatches], numInputs);
 	if (blockSize64 == 0) return PARHIP_OK;
-	size_t in_bytes, out_bytes, cb;
+	size_t in_bytes, out_bytes;
 	RC(words_bytes(numInputs, blockSize64, &in_bytes));
 	RC(words_bytes(numRecovery, blockSize64, &out_bytes));
-	RC(words_bytes(numRecovery, numInputs, &cb));
 	hipStream_t s;
 	RC(host_stream(&s));
 	const size_t slice_bytes = blockSize64 * 8;
-	DevBuf dout, dco;
+	DevBuf dout;
 	RC(dout.alloc(out_bytes, s));
-	RC(dco.alloc(cb, s));
-	RC(parhip_d_gf64_build_cauchy(dco.as<uint64_t>(), numInputs, numRecovery, firstInput,
-	                              firstRecovery, s));
 	if (use_streaming(in_bytes)) {
-		RC(stream_contract(batches_fill(batches, starts, slice_bytes), numInputs, blockSize64,
-		                   dout.as<uint64_t>(), numRecovery, dco.as<uint64_t>(), numInputs, 0, s));
+		RC(stream_cauchy(batches_fill(batches, starts, slice_bytes), numInputs, blockSize64,
+		                 dout.as<uint64_t>(), numRecovery, firstInput, firstRecovery, 0, s));
 	} else {
 		DevBuf din;
 		RC(din.alloc(in_bytes, s));
 		for (size_t b = 0; b < numBatches; b++)
 			if (batchBlocks[b])
 				RC(h2d(din.as<char>() + starts[b] * slice_bytes, batches[b], batchBlocks[b] * slice_bytes, s));
-		RC(parhip_d_gf64_recovery_with_coeff(din.as<uint64_t>(), numInputs, blockSize64,
-		                                     dout.as<uint64_t>(), numRecovery, blockSize64,
-		                                     blockSize64, dco.as<uint64_t>(), 0, s));
+		RC(parhip_d_gf64_recovery(din.as<uint64_t>(), numInputs, blockSize64, dout.as<uint64_t>(),
+		                          numRecovery, blockSize64, blockSize64, firstInput, firstRecovery, 0,
+		                          s));
 	}
 	RC(d2h(recovery, dout.p, out_bytes, s));
 	return sync(s);
@@ -591,11 +602,8 @@ int parhip_par3_create_streaming(const char* sourcePath, int32_t numRecovery, in
 	hipStream_t s = nullptr;
 	if (!rc) rc = host_stream(&s);
 	if (!rc) {
-		DevBuf dout, dco;
+		DevBuf dout;
 		rc = dout.alloc(out_bytes, s);
-		if (!rc) rc = dco.alloc(cb, s);
-		if (!rc)
-			rc = parhip_d_gf64_build_cauchy(dco.as<uint64_t>(), N, R, firstInput, firstRecovery, s);
 		std::atomic<int> read_err{0};
 		const FillFn fill = [&](void* dst, uint64_t first, uint64_t count) {
 			const size_t off = (size_t)first * S, bytes = (size_t)count * S, part = 4u << 20;
@@ -620,7 +628,7 @@ int parhip_par3_create_streaming(const char* sourcePath, int32_t numRecovery, in
 				return fail(PARHIP_ERR_ARG, "read failed: errno=%d (%s)", err, strerror(err));
 			return (int)PARHIP_OK;
 		};
-		if (!rc) rc = stream_contract(fill, N, W, dout.as<uint64_t>(), R, dco.as<uint64_t>(), N, 0, s);
+		if (!rc) rc = stream_cauchy(fill, N, W, dout.as<uint64_t>(), R, firstInput, firstRecovery, 0, s);
 		if (!rc) rc = d2h(rec, dout.p, out_bytes, s);
 		if (!rc) rc = sync(s);
 	}

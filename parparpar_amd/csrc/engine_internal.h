@@ -49,6 +49,27 @@ struct MfmaPlan {
 int plan_mfma(uint64_t N, uint64_t R, uint64_t W, bool fp4, bool overrides, MfmaPlan* p);
 hipError_t enqueue_mfma(const MacArgs& m, const MfmaPlan& p, void* ws, hipStream_t s);
 
+// The XOR-block (dyadic) form of the plain Cauchy contraction (gf64_dyadic.hip).
+struct DyadicPlan {
+	uint64_t pre_in, pre_out, nblocks, Q;
+	int wpt, ib;
+	uint64_t groups, blocks_per_group;
+	size_t part_bytes;  // split-K partials + fold tail (0 when groups == 1)
+};
+DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev);
+// Whether `a` (strided slices, this thread's method) takes the dyadic kernel.
+bool dyadic_applies(const MacArgs& a);
+// gp: the coefficients of the launch's first block; gp_ld: blocks per output
+// group row of the coefficient array.
+hipError_t enqueue_dyadic(const MacArgs& a, const DyadicPlan& p, const uint64_t* gp,
+                          uint64_t gp_ld, void* ws, hipStream_t s);
+// The plain Cauchy contraction of a call with n_total inputs at points
+// (fi + i, fr + r), run over its inputs [first, first + a.N): the dyadic kernel
+// where it applies, else the cached coefficient matrix and run_contraction.
+// a.coeff is ignored; n_total = 0 means a.N.
+int run_cauchy_contraction(MacArgs a, uint64_t fi, uint64_t fr, hipStream_t s, uint64_t first = 0,
+                           uint64_t n_total = 0);
+
 // PAR2 GF(2^16) side (engine_gf16.cpp)
 int gf16_device_log_exp(const uint16_t** exp_tab, const uint16_t** log_tab);
 int gf16_coeffs_device(uint16_t* d_coeff, const uint16_t* d_input_nums, uint64_t N,

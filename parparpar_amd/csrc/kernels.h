@@ -56,6 +56,29 @@ hipError_t launch_batch(const BatchJob* jobs, uint32_t njobs, const BatchTile* t
 
 hipError_t launch_cauchy(uint64_t* out, uint64_t N, uint64_t R, uint64_t fi, uint64_t fr,
                          hipStream_t s);
+
+// Cauchy contraction through its XOR-block structure (gf64_dyadic.hip):
+// virtual input points are firstInput - pre_in + v, v = 8*block + t; virtual
+// recovery points firstRecovery - pre_out + 8*q + s.  gp holds, per (q, block),
+// the 8 coefficients of the block in the subset basis (k_dyadic_coeffs).
+struct DyadicArgs {
+	const uint64_t* in_base;
+	uint64_t in_stride;        // words between consecutive inputs
+	uint64_t* out_base;
+	uint64_t out_stride;
+	const uint64_t* gp;        // coefficients of (q, block b): gp + (q*gp_ld + b)*8
+	uint64_t gp_ld;
+	uint64_t N, R, W;
+	uint64_t pre_in, pre_out;  // firstInput & 7, firstRecovery & 7
+	uint64_t nblocks;          // (pre_in + N + 7) / 8
+	uint64_t blocks_per_group; // split-K: blocks per work-group row
+	uint64_t* partial;         // if non-null: G x R x W partials, reduced afterwards
+	int accumulate;
+};
+hipError_t launch_dyadic_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uint64_t xbase,
+                                uint64_t ybase, hipStream_t s);
+bool mac_dyadic_supported(int wpt, int ib);
+hipError_t launch_mac_dyadic(const DyadicArgs& a, uint64_t groups, int wpt, int ib, hipStream_t s);
 hipError_t launch_cauchy_points(uint64_t* out, const uint64_t* xs, uint64_t nx,
                                 const uint64_t* ys, uint64_t ny, hipStream_t s);
 hipError_t launch_mac_tab(const MacArgs& a, uint64_t groups, int rt, int wpt, int ib,
