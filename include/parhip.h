@@ -67,6 +67,11 @@ int parhip_set_method(int method);
  * thread, ib = inputs per LDS batch; MFMA: rt / wpt = the kernel's two tile
  * dimensions; groups = split-K rows; 0 = automatic. */
 int parhip_tune_override(int rt, int wpt, int ib, int64_t groups);
+/* Coefficient cache of this thread's plain Cauchy calls: on (default) keeps
+ * built coefficient sets on the device in an LRU of 16, as the reference keeps
+ * its Cauchy matrices (src/par3_engine.cc:451-482); off builds them on every
+ * call, into stream-ordered memory freed after the call. */
+int parhip_set_coeff_cache(int enabled);
 /* Release cached device memory (coefficient cache, workspaces, this thread's
  * staging ring). */
 int parhip_release_cache(void);

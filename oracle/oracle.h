@@ -74,6 +74,11 @@ void or_par2_recovery_columns(const uint8_t *inputs, size_t numInputs, size_t sl
                               size_t numOutputs, const uint64_t *cols, size_t ncols,
                               uint16_t *out);
 
+/* CPU baseline of the PAR2 create (baseline_gf16.c): out = r x words;
+ * returns the threads used.  Checked against or_par2_create in the tests. */
+int bl_par2_create(const uint16_t *in, size_t n, size_t words, const uint16_t *input_nums,
+                   const uint16_t *exps, size_t r, uint16_t *out, int threads);
+
 /* ---------------- PAR3 recovery packets + BLAKE3 seal ---------------- */
 void or_blake3_compress(const uint32_t cv[8], const uint32_t block[16], uint64_t counter,
                         uint32_t block_len, uint32_t flags, uint32_t out[16]);

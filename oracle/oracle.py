@@ -80,6 +80,8 @@ def lib():
         L.or_gf16_rec_matrix_compute.restype = ctypes.c_int
         L.or_gf16_rec_matrix_compute.argtypes = [_u8p, _sz, _sz, _u16p, ctypes.POINTER(_sz), _u16p]
         L.or_assemble_recovery_packets.argtypes = [_u8p, _u8p, _u64, _sz, _sz, _u8p]
+        L.bl_par2_create.restype = ctypes.c_int
+        L.bl_par2_create.argtypes = [_u16p, _sz, _sz, _u16p, _u16p, _sz, _u16p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -265,3 +267,16 @@ def gf16_rec_matrix(input_valid, recovery):
     ok = lib().or_gf16_rec_matrix_compute(valid.ctypes.data_as(_u8p), n, vc, p16(rec),
                                           ctypes.byref(cnt), p16(out))
     return bool(ok), rec[:cnt.value].copy(), out
+
+
+def baseline_par2_create(inputs, input_nums, exponents, threads):
+    """CPU baseline of the PAR2 create (baseline_gf16.c): inputs (N, words)
+    uint16; returns ((R, words) uint16, threads used).  bench.py times it."""
+    x = np.ascontiguousarray(inputs, dtype=np.uint16)
+    n, words = x.shape
+    inums = np.ascontiguousarray(input_nums, dtype=np.uint16)
+    exps = np.ascontiguousarray(exponents, dtype=np.uint16)
+    out = np.zeros((exps.size, words), dtype=np.uint16)
+    used = lib().bl_par2_create(p16(x), n, words, p16(inums), p16(exps), exps.size, p16(out),
+                                int(threads))
+    return out, used

@@ -30,6 +30,7 @@ _SIGS = [
     ("parhip_set_method", _int, [_int]),
     ("parhip_release_cache", _int, []),
     ("parhip_set_stream_chunk", _int, [_sz]),
+    ("parhip_set_coeff_cache", _int, [_int]),
     ("parhip_gf16_rec_matrix_compute", _int,
      [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.POINTER(ctypes.c_uint32), _vp]),
     ("parhip_d_gf16_rec_matrix_compute", _int,

@@ -5,9 +5,11 @@
 // ComputeRecoveryBlocksWithCoeff, src/par3_engine.cc:1165); here a batch of
 // jobs is planned once into a device queue of tiles (job, word range, output
 // range, input range), each tile using the launch variant the autotuner chose
-// for its job's shape, and the whole sequence -- every job's Cauchy matrix,
+// for its job's shape, and the whole sequence -- every job's coefficients,
 // zeroing of split jobs' outputs, queue reset, persistent kernel -- is
-// captured in a hipGraph and replayed.  Jobs whose input range is split over
+// captured in a hipGraph and replayed.  Jobs whose slices allow it run as
+// XOR-block (dyadic) tiles, the same method as the plain create
+// (gf64_dyadic_dev.h); the others on the nibble-table tiles.  Jobs whose input range is split over
 // several tiles accumulate with 64-bit atomic XOR (order-independent: the
 // bytes are identical to the unsplit computation).  A job can instead run on
 // the GF(2) bit-matrix (MFMA) strategy, as its own launch sequence after the
@@ -38,7 +40,22 @@ struct CfgInfo {
 // batch; MFMA: outputs per work-group, 32-word column blocks per wave, inputs
 // per stage of the fp4 kernel's default shape
 static const CfgInfo kCfg[BATCH_CFG_COUNT] = {
-	{8, 4, 4}, {8, 2, 4}, {4, 2, 4} /* MFMA */, {16, 2, 2}, {4, 4, 4}};
+	{8, 4, 4}, {8, 2, 4}, {4, 2, 4} /* MFMA */, {16, 2, 2}, {4, 4, 4},
+	{8, 4, 1} /* dyadic, one 8-output group per tile */, {8, 2, 1} /* dyadic */};
+
+static bool is_dyadic(int cfg) { return cfg == BATCH_CFG_DY4 || cfg == BATCH_CFG_DY2; }
+
+// XOR-block tiles run the straight-line (full, 16-byte aligned tile) form of
+// the kernel: slices a multiple of 512 words, inputs 16-byte aligned at an
+// even stride; 27 products per 8 x 8 block pay off from 4 outputs up (the
+// rule of the plain create).
+static bool dyadic_ok(const BatchJob& jb) {
+	return jb.R >= 4 && jb.W % 512 == 0 && (reinterpret_cast<uintptr_t>(jb.in) & 15u) == 0 &&
+	       (jb.in_stride & 1) == 0;
+}
+static int dyadic_default(const BatchJob& jb) {
+	return jb.W % 1024 == 0 ? BATCH_CFG_DY4 : BATCH_CFG_DY2;
+}
 
 // The MFMA strategy needs the 64 bit-matrix rows of every coefficient (512 B
 // each) beside the job; it is offered only where they are no larger than the
@@ -77,8 +94,37 @@ static std::vector<int> candidates(uint64_t R, uint64_t W) {
 // ---------------------------------------------------------------- tile planning
 static const uint64_t kTileWork = 1ull << 22;  // word x output MACs per tile
 
+// XOR-block tiles: a block is 8 inputs x 8 outputs per word; a tile is
+// about twice a table tile's MACs (the method takes ~half the time per MAC).
+static uint32_t plan_job_dyadic(uint32_t jid, const BatchJob& jb, int cfg,
+                                std::vector<BatchTile>& out, bool* atomic) {
+	const uint64_t tw = 256ull * kCfg[cfg].wpt;
+	const uint64_t wt = jb.W / tw;
+	uint64_t chunk = std::max<uint64_t>(2, (2 * kTileWork) / (tw * 64));
+	if (chunk >= jb.nblocks) chunk = jb.nblocks;
+	const uint64_t nch = (jb.nblocks + chunk - 1) / chunk;
+	*atomic = nch > 1;
+	uint32_t n = 0;
+	for (uint64_t k = 0; k < nch; k++)
+		for (uint64_t q = 0; q < jb.Q; q++)
+			for (uint64_t w = 0; w < wt; w++) {
+				BatchTile t;
+				t.job = jid;
+				t.cfg = (uint16_t)cfg;
+				t.mode = *atomic ? 2 : 0;
+				t.r0 = q;
+				t.w0 = w * tw;
+				t.i0 = k * chunk;
+				t.i1 = std::min<uint64_t>(jb.nblocks, t.i0 + chunk);
+				out.push_back(t);
+				n++;
+			}
+	return n;
+}
+
 static uint32_t plan_job(uint32_t jid, const BatchJob& jb, int cfg, std::vector<BatchTile>& out,
                          bool* atomic) {
+	if (is_dyadic(cfg)) return plan_job_dyadic(jid, jb, cfg, out, atomic);
 	const CfgInfo& c = kCfg[cfg];
 	const uint64_t tw = 256ull * c.wpt;
 	const uint64_t wt = (jb.W + tw - 1) / tw;
@@ -112,7 +158,10 @@ static uint32_t plan_job(uint32_t jid, const BatchJob& jb, int cfg, std::vector<
 // job's own buffers and cached by (words per slice, R, input-count bucket).
 struct TuneKey {
 	uint64_t W, R, nb;
-	bool operator<(const TuneKey& o) const { return std::tie(W, R, nb) < std::tie(o.W, o.R, o.nb); }
+	int dy;  // the job qualifies for XOR-block tiles
+	bool operator<(const TuneKey& o) const {
+		return std::tie(W, R, nb, dy) < std::tie(o.W, o.R, o.nb, o.dy);
+	}
 };
 struct TuneVal {
 	int cfg;
@@ -145,7 +194,7 @@ static int run_single(const BatchJob& jb, int cfg, BatchJob* djob, BatchTile* dt
 	for (int rep = 0; rep < 3; rep++) {
 		hipEventRecord(a, s);
 		hipError_t e = launch_batch(djob, 1, dtiles, (uint32_t)tiles.size(), dcounter, dzero,
-		                            atomic ? 1 : 0, 0, blocks, s);
+		                            atomic ? 1 : 0, 0, 0, blocks, s);
 		hipEventRecord(b, s);
 		if (e != hipSuccess) {
 			(void)hipStreamSynchronize(s);  // the tile copy above reads `tiles`
@@ -164,9 +213,12 @@ static int run_single(const BatchJob& jb, int cfg, BatchJob* djob, BatchTile* dt
 	return PARHIP_OK;
 }
 
-// Tune (or look up) the variant for a job; `jb.coeff` must hold its matrix.
+// Tune (or look up) the variant for a job; `jb.coeff` must hold its matrix
+// (table jobs, MFMA candidates) and `jb.gp` its XOR-block coefficients
+// (dyadic jobs).
 static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_out) {
-	TuneKey key{jb.W, jb.R, n_bucket(jb.N)};
+	const bool dy = dyadic_ok(jb);
+	TuneKey key{jb.W, jb.R, n_bucket(jb.N), dy ? 1 : 0};
 	{
 		std::lock_guard<std::mutex> lk(g_tune_mu);
 		auto it = g_tune.find(key);
@@ -176,6 +228,13 @@ static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_
 		}
 	}
 	std::vector<int> cand = candidates(jb.R, jb.W);
+	if (dy) {
+		// the XOR-block method does ~half the LDS work of the direct tables
+		// per MAC at every R measured (4-64); only its tile width is tuned
+		cand.clear();
+		if (jb.W % 1024 == 0) cand.push_back(BATCH_CFG_DY4);
+		cand.push_back(BATCH_CFG_DY2);
+	}
 	const bool try_mfma = mfma_eligible(jb);
 	if (cand.size() == 1 && !try_mfma) {
 		*cfg_out = cand[0];
@@ -223,6 +282,12 @@ static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_
 		DevBuf mws;
 		if ((rc = mws.alloc(mp.rows_bytes + mp.part_bytes, s))) return rc;
 		const MacArgs m = mfma_args(jb, dout.as<uint64_t>(), jb.W);
+		DyadicPlan dp{};
+		DevBuf dws;
+		if (dy) {
+			dp = plan_dyadic(jb.N, jb.R, jb.W, jb.fi, jb.fr, current_device());
+			if ((rc = dws.alloc(dp.part_bytes, s))) return rc;
+		}
 		float ms_tab = 1e30f, ms_mfma = 1e30f;
 		hipEvent_t a, b;
 		hipEventCreate(&a);
@@ -232,7 +297,9 @@ static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_
 		for (int which = 0; which < 2 && !rc && e == hipSuccess; which++)
 			for (int rep = 0; rep < 3 && !rc && e == hipSuccess; rep++) {
 				hipEventRecord(a, s);
-				if (which == 0)
+				if (which == 0 && dy)
+					e = enqueue_dyadic(m, dp, jb.gp, jb.nblocks, dws.p, s);
+				else if (which == 0)
 					rc = run_contraction_tables(m, s);
 				else
 					e = enqueue_mfma(m, mp, mws.p, s);
@@ -277,7 +344,9 @@ struct parhip_batch {
 	uint32_t* dzero = nullptr;
 	uint32_t nzero = 0;
 	uint64_t* dcoeff = nullptr;
-	uint64_t coeff_total = 0;
+	uint64_t coeff_total = 0;   // full matrices built per run (table / MFMA jobs)
+	uint64_t* dgp = nullptr;
+	uint64_t dy_pairs = 0;      // XOR-block coefficient pairs built per run (dyadic jobs)
 	int blocks = 0;
 	hipStream_t cap = nullptr;
 	hipGraph_t graph = nullptr;
@@ -295,7 +364,7 @@ struct parhip_batch {
 // jobs, queue reset and persistent kernel, then the MFMA jobs' launches.
 static hipError_t batch_enqueue(const parhip_batch* b, hipStream_t s) {
 	hipError_t e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), b->dtiles, b->ntiles, b->dcounter,
-	                            b->dzero, b->nzero, b->coeff_total, b->blocks, s);
+	                            b->dzero, b->nzero, b->coeff_total, b->dy_pairs, b->blocks, s);
 	for (size_t k = 0; k < b->mfma.size() && e == hipSuccess; k++) {
 		const auto& mj = b->mfma[k];
 		const BatchJob& jb = b->jobs[mj.job];
@@ -315,6 +384,7 @@ static void batch_free(parhip_batch* b) {
 	if (b->dcounter) hipFree(b->dcounter);
 	if (b->dzero) hipFree(b->dzero);
 	if (b->dcoeff) hipFree(b->dcoeff);
+	if (b->dgp) hipFree(b->dgp);
 	for (auto& mj : b->mfma)
 		if (mj.ws) hipFree(mj.ws);
 	if (b->cap) hipStreamDestroy(b->cap);
@@ -361,13 +431,25 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 		jb.W = in.blockSize64;
 		jb.fi = in.firstInput;
 		jb.fr = in.firstRecovery;
+		jb.pre_in = jb.fi & 7;
+		jb.pre_out = jb.fr & 7;
+		jb.nblocks = (jb.pre_in + jb.N + 7) / 8;
+		jb.Q = (jb.pre_out + jb.R + 7) / 8;
+		// storage for every form the job may run in: XOR-block coefficients
+		// where it qualifies, the full matrix where tables or MFMA may run it
+		const bool dy = dyadic_ok(jb);
+		const bool full = !dy || ((flags & (PARHIP_BATCH_MFMA | PARHIP_BATCH_AUTOTUNE)) && mfma_eligible(jb));
 		jb.coeff_off = b->coeff_total;
-		b->coeff_total += jb.N * jb.R;
+		if (full) b->coeff_total += jb.N * jb.R;
+		jb.gp_off = b->dy_pairs;
+		if (dy) b->dy_pairs += jb.Q * jb.nblocks;
 		b->slot[j] = (int64_t)b->jobs.size();
 		b->jobs.push_back(jb);
 	}
 	if (b->jobs.empty()) return b;  // nothing to do: run() is a no-op
-	e = hipMalloc((void**)&b->dcoeff, b->coeff_total * sizeof(uint64_t));
+	e = hipSuccess;
+	if (b->coeff_total) e = hipMalloc((void**)&b->dcoeff, b->coeff_total * sizeof(uint64_t));
+	if (e == hipSuccess && b->dy_pairs) e = hipMalloc((void**)&b->dgp, b->dy_pairs * 8 * sizeof(uint64_t));
 	if (e == hipSuccess) e = hipMalloc((void**)&b->djobs, b->jobs.size() * sizeof(BatchJob));
 	if (e == hipSuccess) e = hipMalloc((void**)&b->dcounter, sizeof(uint32_t));
 	if (e != hipSuccess) {
@@ -375,12 +457,15 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 		batch_free(b);
 		return nullptr;
 	}
-	for (auto& jb : b->jobs) jb.coeff = b->dcoeff + jb.coeff_off;
+	for (auto& jb : b->jobs) {
+		jb.coeff = b->dcoeff ? b->dcoeff + jb.coeff_off : nullptr;
+		jb.gp = b->dgp ? b->dgp + jb.gp_off * 8 : nullptr;
+	}
 	e = hipMemcpy(b->djobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice);
-	// coefficients first: the autotuner runs the real kernel on them
+	// coefficients first: the autotuner runs the real kernels on them
 	if (e == hipSuccess)
 		e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), nullptr, 0, b->dcounter, nullptr, 0,
-		                 b->coeff_total, 0, b->cap);
+		                 b->coeff_total, b->dy_pairs, 0, b->cap);
 	if (e == hipSuccess) e = hipStreamSynchronize(b->cap);
 	if (e != hipSuccess) {
 		hip_fail(e, "batch coefficients");
@@ -392,6 +477,7 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 	for (size_t j = 0; j < b->jobs.size(); j++) {
 		const BatchJob& jb = b->jobs[j];
 		int cfg = jb.R <= 4 ? BATCH_CFG_4_4_4 : (jb.W <= 512 ? BATCH_CFG_8_2_4 : BATCH_CFG_8_4_4);
+		if (dyadic_ok(jb)) cfg = dyadic_default(jb);
 		if ((flags & PARHIP_BATCH_MFMA) && mfma_eligible(jb)) {
 			cfg = BATCH_CFG_MFMA;
 		} else if (flags & PARHIP_BATCH_AUTOTUNE) {
@@ -415,6 +501,28 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 			b->mfma.push_back(mj);
 		}
 		b->cfg[j] = cfg;
+	}
+	// ---- per-run coefficient layout: full matrices only for the table and
+	// MFMA jobs, XOR-block coefficients only for the dyadic jobs
+	b->coeff_total = 0;
+	b->dy_pairs = 0;
+	for (size_t j = 0; j < b->jobs.size(); j++) {
+		BatchJob& jb = b->jobs[j];
+		jb.coeff_off = b->coeff_total;
+		jb.gp_off = b->dy_pairs;
+		if (is_dyadic(b->cfg[j])) {
+			b->dy_pairs += jb.Q * jb.nblocks;
+		} else {
+			jb.coeff = b->dcoeff + jb.coeff_off;
+			b->coeff_total += jb.N * jb.R;
+		}
+		if (jb.gp) jb.gp = b->dgp + jb.gp_off * 8;
+	}
+	e = hipMemcpy(b->djobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice);
+	if (e != hipSuccess) {
+		hip_fail(e, "batch job upload");
+		batch_free(b);
+		return nullptr;
 	}
 	// ---- tiles: interleave jobs so concurrent work-groups mix shapes
 	std::vector<std::vector<BatchTile>> per(b->jobs.size());
@@ -495,7 +603,7 @@ int parhip_batch_info(const parhip_batch* b, uint64_t* ntiles, uint64_t* natomic
 	if (!b) return fail(PARHIP_ERR_ARG, "batch_info: null batch");
 	if (ntiles) *ntiles = b->ntiles;
 	if (natomic_jobs) *natomic_jobs = b->nzero;
-	if (coeff_bytes) *coeff_bytes = b->coeff_total * sizeof(uint64_t);
+	if (coeff_bytes) *coeff_bytes = (b->coeff_total + b->dy_pairs * 8) * sizeof(uint64_t);
 	return PARHIP_OK;
 }
 
@@ -512,15 +620,25 @@ int parhip_batch_job_config(const parhip_batch* b, size_t job, int* rt, int* wpt
 int parhip_batch_job_method(const parhip_batch* b, size_t job) {
 	if (!b || job >= b->slot.size()) return fail(PARHIP_ERR_ARG, "batch_job_method: bad job");
 	if (b->slot[job] < 0) return fail(PARHIP_ERR_ARG, "batch_job_method: job %zu is empty", job);
-	return b->cfg[(size_t)b->slot[job]] == BATCH_CFG_MFMA ? PARHIP_METHOD_MFMA : PARHIP_METHOD_TABLE;
+	const int cfg = b->cfg[(size_t)b->slot[job]];
+	return cfg == BATCH_CFG_MFMA ? PARHIP_METHOD_MFMA
+	                             : (is_dyadic(cfg) ? PARHIP_METHOD_DYADIC : PARHIP_METHOD_TABLE);
 }
 
 void parhip_batch_destroy(parhip_batch* b) { batch_free(b); }
 
+// The entry of a shape: the XOR-block one where the shape qualifies for it
+// (aligned slices of a multiple of 512 words, R >= 4), else the other.
+static std::map<TuneKey, TuneVal>::iterator tune_find(size_t N, size_t R, size_t W) {
+	auto it = g_tune.find(TuneKey{W, R, n_bucket(N), 1});
+	if (it == g_tune.end()) it = g_tune.find(TuneKey{W, R, n_bucket(N), 0});
+	return it;
+}
+
 int parhip_autotune_lookup(size_t numInputs, size_t numRecovery, size_t blockSize64, int* rt,
                            int* wpt, int* ib, double* ns_per_mac) {
 	std::lock_guard<std::mutex> lk(g_tune_mu);
-	auto it = g_tune.find(TuneKey{blockSize64, numRecovery, n_bucket(numInputs)});
+	auto it = tune_find(numInputs, numRecovery, blockSize64);
 	if (it == g_tune.end()) return fail(PARHIP_ERR_ARG, "autotune: shape not tuned");
 	const CfgInfo& c = kCfg[it->second.cfg];
 	if (rt) *rt = c.rt;
@@ -532,9 +650,11 @@ int parhip_autotune_lookup(size_t numInputs, size_t numRecovery, size_t blockSiz
 
 int parhip_autotune_lookup_method(size_t numInputs, size_t numRecovery, size_t blockSize64) {
 	std::lock_guard<std::mutex> lk(g_tune_mu);
-	auto it = g_tune.find(TuneKey{blockSize64, numRecovery, n_bucket(numInputs)});
+	auto it = tune_find(numInputs, numRecovery, blockSize64);
 	if (it == g_tune.end()) return fail(PARHIP_ERR_ARG, "autotune: shape not tuned");
-	return it->second.cfg == BATCH_CFG_MFMA ? PARHIP_METHOD_MFMA : PARHIP_METHOD_TABLE;
+	const int cfg = it->second.cfg;
+	return cfg == BATCH_CFG_MFMA ? PARHIP_METHOD_MFMA
+	                             : (is_dyadic(cfg) ? PARHIP_METHOD_DYADIC : PARHIP_METHOD_TABLE);
 }
 
 int parhip_autotune_save(const char* path) {
@@ -542,11 +662,11 @@ int parhip_autotune_save(const char* path) {
 	FILE* f = fopen(path, "w");
 	if (!f) return fail(PARHIP_ERR_ARG, "autotune_save: cannot open %s", path);
 	std::lock_guard<std::mutex> lk(g_tune_mu);
-	fprintf(f, "# words_per_slice recovery input_bucket cfg ns_per_mac\n");
+	fprintf(f, "# words_per_slice recovery input_bucket xor_block cfg ns_per_mac\n");
 	for (auto& kv : g_tune)
-		fprintf(f, "%llu %llu %llu %d %.6g\n", (unsigned long long)kv.first.W,
-		        (unsigned long long)kv.first.R, (unsigned long long)kv.first.nb, kv.second.cfg,
-		        kv.second.ns_per_mac);
+		fprintf(f, "%llu %llu %llu %d %d %.6g\n", (unsigned long long)kv.first.W,
+		        (unsigned long long)kv.first.R, (unsigned long long)kv.first.nb, kv.first.dy,
+		        kv.second.cfg, kv.second.ns_per_mac);
 	fclose(f);
 	return PARHIP_OK;
 }
@@ -560,11 +680,12 @@ int parhip_autotune_load(const char* path) {
 	while (fgets(line, sizeof line, f)) {
 		if (line[0] == '#') continue;
 		unsigned long long W, R, nb;
-		int cfg;
+		int dy, cfg;
 		double ns;
-		if (sscanf(line, "%llu %llu %llu %d %lg", &W, &R, &nb, &cfg, &ns) == 5 && cfg >= 0 &&
-		    cfg < BATCH_CFG_COUNT && kCfg[cfg].rt)
-			g_tune[TuneKey{W, R, nb}] = {cfg, ns};
+		// an XOR-block variant only under a key that qualifies for it
+		if (sscanf(line, "%llu %llu %llu %d %d %lg", &W, &R, &nb, &dy, &cfg, &ns) == 6 && cfg >= 0 &&
+		    cfg < BATCH_CFG_COUNT && kCfg[cfg].rt && (dy == 0 || dy == 1) && (dy || !is_dyadic(cfg)))
+			g_tune[TuneKey{W, R, nb, dy}] = {cfg, ns};
 	}
 	fclose(f);
 	return PARHIP_OK;

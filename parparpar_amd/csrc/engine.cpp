@@ -162,8 +162,37 @@ int coeff_acquire(uint64_t n, uint64_t r, uint64_t fi, uint64_t fr, hipStream_t 
 	return coeff_acquire_kind(0, n, r, fi, fr, s, out);
 }
 
+// Build a coefficient set of `kind` into `d` on stream s.
+static hipError_t coeff_build(int kind, uint64_t* d, uint64_t n, uint64_t r, uint64_t fi,
+                              uint64_t fr, hipStream_t s) {
+	if (kind) {
+		ProfScope ps("dyadic_coeffs", s);
+		return launch_dyadic_coeffs(d, ((fi & 7) + n + 7) / 8, ((fr & 7) + r + 7) / 8, fi & ~7ull,
+		                            fr & ~7ull, s);
+	}
+	ProfScope ps("cauchy", s);
+	return launch_cauchy(d, n, r, fi, fr, s);
+}
+
+// parhip_set_coeff_cache: off = every call builds its own coefficients into
+// stream-ordered memory, freed by coeff_release.
+static thread_local bool t_coeff_cache = true;
+
 static int coeff_acquire_kind(int kind, uint64_t n, uint64_t r, uint64_t fi, uint64_t fr,
                               hipStream_t s, const uint64_t** out) {
+	const uint64_t words = kind ? dyadic_coeff_words(n, r, fi, fr) : n * r;
+	if (!t_coeff_cache) {
+		void* p = nullptr;
+		int rc = dev_alloc(&p, words * sizeof(uint64_t), s);
+		if (rc) return rc;
+		hipError_t e = coeff_build(kind, (uint64_t*)p, n, r, fi, fr, s);
+		if (e != hipSuccess) {
+			dev_free(p, s);
+			return hip_fail(e, kind ? "k_dyadic_coeffs" : "k_cauchy");
+		}
+		*out = (const uint64_t*)p;
+		return PARHIP_OK;
+	}
 	CoeffKey key{t_device, kind, n, r, fi, fr};
 	std::unique_lock<std::mutex> lk(g_coeff_mu);
 	auto it = g_coeff.find(key);
@@ -193,17 +222,9 @@ static int coeff_acquire_kind(int kind, uint64_t n, uint64_t r, uint64_t fi, uin
 		if (!evicted) break;
 	}
 	CoeffEntry ent;
-	const uint64_t words = kind ? dyadic_coeff_words(n, r, fi, fr) : n * r;
 	hipError_t e = hipMalloc((void**)&ent.d, words * sizeof(uint64_t));
 	if (e != hipSuccess) return hip_fail(e, "hipMalloc(coefficients)");
-	if (kind) {
-		ProfScope ps("dyadic_coeffs", s);
-		e = launch_dyadic_coeffs(ent.d, ((fi & 7) + n + 7) / 8, ((fr & 7) + r + 7) / 8, fi & ~7ull,
-		                         fr & ~7ull, s);
-	} else {
-		ProfScope ps("cauchy", s);
-		e = launch_cauchy(ent.d, n, r, fi, fr, s);
-	}
+	e = coeff_build(kind, ent.d, n, r, fi, fr, s);
 	if (e != hipSuccess) {
 		hipFree(ent.d);
 		return hip_fail(e, kind ? "k_dyadic_coeffs" : "k_cauchy");
@@ -217,13 +238,17 @@ static int coeff_acquire_kind(int kind, uint64_t n, uint64_t r, uint64_t fi, uin
 	return PARHIP_OK;
 }
 
-void coeff_release(const uint64_t* d) {
-	std::lock_guard<std::mutex> lk(g_coeff_mu);
-	for (auto& kv : g_coeff)
-		if (kv.second.d == d) {
-			kv.second.refs--;
-			return;
-		}
+void coeff_release(const uint64_t* d, hipStream_t s) {
+	{
+		std::lock_guard<std::mutex> lk(g_coeff_mu);
+		for (auto& kv : g_coeff)
+			if (kv.second.d == d) {
+				kv.second.refs--;
+				return;
+			}
+	}
+	// not a cache entry (entries in use are never evicted): built for one call
+	dev_free((void*)d, s);
 }
 
 void coeff_cache_clear() {
@@ -625,13 +650,13 @@ int run_cauchy_contraction(MacArgs a, uint64_t fi, uint64_t fr, hipStream_t s, u
 		void* ws = nullptr;
 		rc = dev_alloc(&ws, p.part_bytes, s);
 		if (rc) {
-			coeff_release(gp);
+			coeff_release(gp, s);
 			return rc;
 		}
 		const uint64_t call_blocks = ((fi & 7) + n_total + 7) / 8;
 		const hipError_t e = enqueue_dyadic(a, p, gp + b0 * 8, call_blocks, ws, s);
 		dev_free(ws, s);
-		coeff_release(gp);
+		coeff_release(gp, s);
 		return e == hipSuccess ? PARHIP_OK : hip_fail(e, "k_mac_dyadic");
 	}
 	const uint64_t* coeff = nullptr;
@@ -640,7 +665,7 @@ int run_cauchy_contraction(MacArgs a, uint64_t fi, uint64_t fr, hipStream_t s, u
 	a.coeff = coeff + first;
 	a.coeff_ld = n_total;
 	rc = run_contraction(a, s);
-	coeff_release(coeff);
+	coeff_release(coeff, s);
 	return rc;
 }
 
@@ -702,6 +727,11 @@ int parhip_tune_override(int rt, int wpt, int ib, int64_t groups) {
 	t_tune_wpt = wpt;
 	t_tune_ib = ib;
 	t_tune_groups = groups;
+	return PARHIP_OK;
+}
+
+int parhip_set_coeff_cache(int enabled) {
+	t_coeff_cache = enabled != 0;
 	return PARHIP_OK;
 }
 

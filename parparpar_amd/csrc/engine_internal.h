@@ -28,7 +28,8 @@ int dev_alloc(void** p, size_t bytes, hipStream_t s);
 void dev_free(void* p, hipStream_t s);
 int coeff_acquire(uint64_t n, uint64_t r, uint64_t fi, uint64_t fr, hipStream_t s,
                   const uint64_t** out);
-void coeff_release(const uint64_t* d);
+// s: the stream of the call (coefficients built for that call alone are freed on it)
+void coeff_release(const uint64_t* d, hipStream_t s);
 void coeff_cache_clear();
 Plan plan_contraction(uint64_t N, uint64_t R, uint64_t W, int dev);
 size_t plan_workspace_bytes(const Plan& p, uint64_t R, uint64_t W);

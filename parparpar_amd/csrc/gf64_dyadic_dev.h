@@ -1,0 +1,335 @@
+// gf64_dyadic_dev.h -- device side of the XOR-block (dyadic) Cauchy
+// contraction, shared by the plain kernel (gf64_dyadic.hip) and the persistent
+// batch kernel (gf64_kernels.hip).
+//
+// The plain create multiplies by the Cauchy matrix of src/par3_engine.cc:358-429,
+//   c(r, i) = f((firstInput + i) XOR (firstRecovery + r)),   f(z) = z ? 1/z : 1.
+// Group the input points X = firstInput + i and the recovery points
+// Y = firstRecovery + r into aligned runs of 8: X = 8b + t, Y = 8q + s with
+// t, s in 0..7.  Then X XOR Y = (8b XOR 8q) XOR (t XOR s), so inside every
+// 8 x 8 block (b, q) the coefficient depends on t XOR s only:
+//   c = g_bq(t XOR s).
+// Such a block is multiplication in the group algebra GF(2^64)[Z_2^3]
+// (y_s = XOR_t g(t XOR s) d_t is the group convolution g * d).  In
+// characteristic 2 that algebra is GF(2^64)[u1,u2,u3]/(u_k^2) with
+// u_k = 1 + t_k, and in the u basis
+//   a'_T = XOR_{S superset of T} a_S          (its own inverse: the same sums)
+//   y'_U = XOR_{S, T disjoint, S | T = U} g'_S * d'_T     (subset convolution)
+// which needs 3^3 = 27 field products per block instead of 8 x 8 = 64.  The
+// transforms are XORs of words; the result is the same linear map, so the
+// bytes are exactly those of the full product.  Outputs stay in the u basis
+// over all input blocks (the back transform is linear and the same for every
+// block) and are transformed once before the store.
+//
+// Products still go through nibble tables in LDS, two coefficients per 16-byte
+// entry (one ds_read_b128 gives two products of one input nibble, and a
+// 16-entry table is one 256-byte row: conflict-free).  Input d'_T needs the
+// coefficients g'_S with S disjoint from T; they are served by seven pair
+// tables per block:
+//   P0 (0,1) P2 (2,3) P4 (4,5) P6 (6,7) Q0 (0,2) Q4 (4,6) X (0,4)
+//   T = 0: P0 P2 P4 P6   T = 1: Q0 Q4   T = 2: P0 P4   T = 3: X
+//   T = 4: P0 P2         T = 5: Q0      T = 6: P0      T = 7: first half of Q0
+// so a word of a block costs 13 x 16 ds_read_b128 + 16 ds_read_b64 (3.5 KiB of
+// LDS reads) against 32 x 16 ds_read_b128 (8 KiB) for the direct form.
+//
+// Points that are not multiples of 8 are handled with virtual points: the
+// first block (and first output group) is padded in front with inputs that
+// hold zeros (rows that are not stored), so every plain Cauchy call qualifies.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gf64_dev.h"
+#include "kernels.h"
+
+namespace parhip {
+
+
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(1))) const uint64_t dy_gu64;
+typedef __attribute__((address_space(1))) const ulonglong2 dy_gu128;
+__device__ __forceinline__ uint64_t dy_load(const uint64_t* p) { return *(dy_gu64*)p; }
+__device__ __forceinline__ ulonglong2 dy_load2(const uint64_t* p) {
+	dy_gu128* q = (dy_gu128*)p;
+	ulonglong2 v;
+	v.x = q->x;
+	v.y = q->y;
+	return v;
+}
+#else
+__device__ __forceinline__ uint64_t dy_load(const uint64_t* p) { return *p; }
+__device__ __forceinline__ ulonglong2 dy_load2(const uint64_t* p) {
+	return *reinterpret_cast<const ulonglong2*>(p);
+}
+#endif
+
+__device__ __forceinline__ uint32_t dy_xor3(uint32_t a, uint32_t b, uint32_t c) {
+	return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// The seven pair tables: coefficient subsets (slot a, slot b).
+__device__ constexpr int kDySA[7] = {0, 2, 4, 6, 0, 4, 0};
+__device__ constexpr int kDySB[7] = {1, 3, 5, 7, 2, 6, 4};
+// Tables read for input T (T = 7 reads slot a of table 4 only).
+__device__ constexpr int kDyNT[8] = {4, 2, 2, 1, 2, 1, 1, 1};
+__device__ constexpr int kDyTab[8][4] = {{0, 1, 2, 3}, {4, 5, 0, 0}, {0, 2, 0, 0}, {6, 0, 0, 0},
+                                         {0, 1, 0, 0}, {4, 0, 0, 0}, {0, 0, 0, 0}, {4, 0, 0, 0}};
+constexpr int kDyTables = 7;
+
+// a'_T = XOR over supersets S of T of a_S (3-bit masks), in place; also the
+// inverse transform.
+__device__ __forceinline__ void dy_superset(uint2 (&x)[8]) {
+#pragma unroll
+	for (int bit = 1; bit < 8; bit <<= 1)
+#pragma unroll
+		for (int s = 0; s < 8; s++)
+			if (!(s & bit)) {
+				x[s].x ^= x[s | bit].x;
+				x[s].y ^= x[s | bit].y;
+			}
+}
+
+// Byte offsets 16 * n_k of the 8 nibbles n_k of a 32-bit word (table rows of
+// 16-byte entries): SDWA byte selects, one VALU op per offset.
+//   odd nibble 2m+1:  byte_m & 0xF0
+//   even nibble 2m:   (byte_m << 4) & 0xFF   (dst_sel BYTE_0, upper bits zeroed)
+__device__ __forceinline__ void dy_nib16x8(uint32_t x, uint32_t (&ad)[8]) {
+#define DY_HI(M, R) asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_" #M " src1_sel:DWORD" : "=v"(R) : "v"(x), "s"(0xF0u))
+#define DY_LO(M, R) asm("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:BYTE_0 dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_" #M : "=v"(R) : "v"(x))
+	DY_LO(0, ad[0]);
+	DY_HI(0, ad[1]);
+	DY_LO(1, ad[2]);
+	DY_HI(1, ad[3]);
+	DY_LO(2, ad[4]);
+	DY_HI(2, ad[5]);
+	DY_LO(3, ad[6]);
+	DY_HI(3, ad[7]);
+#undef DY_HI
+#undef DY_LO
+}
+
+// The two coefficients of the table entries this thread builds for block b
+// (thread (tbl, n) with tbl < 7), 0 past the group's blocks.
+__device__ __forceinline__ void dy_load_coeffs(const DyadicArgs& a, uint64_t q, uint64_t b,
+                                               uint64_t b_end, uint64_t& c0, uint64_t& c1) {
+	const int tid = threadIdx.x;
+	c0 = 0;
+	c1 = 0;
+	if (tid < kDyTables * 16 && b < b_end) {
+		const int tbl = tid >> 4;
+		const uint64_t* gp = a.gp + (q * a.gp_ld + b) * 8;
+		c0 = dy_load(gp + kDySA[tbl]);
+		c1 = dy_load(gp + kDySB[tbl]);
+	}
+}
+
+// This thread's word pair JJ (words w0 + 2*tid + 512*JJ, +1) of the 8 inputs of
+// block b: zeros for virtual inputs, past the group's blocks or past the slice
+// end.  Indices are clamped into range and out-of-range words replaced by zeros
+// with selects, so the loads are straight-line code; FAST (full, 16-byte
+// aligned tiles) loads each pair with one 16-byte load.
+template <int WPT, int JJ, bool FAST>
+__device__ __forceinline__ void dy_load_pair(const DyadicArgs& a, uint64_t b, uint64_t b_end,
+                                             uint64_t w0, uint2 (&d)[8][WPT]) {
+	const int tid = threadIdx.x;
+	const uint64_t w = w0 + 2 * tid + 512 * JJ;
+#pragma unroll
+	for (int t = 0; t < 8; t++) {
+		// real input of virtual slot t of block b
+		const uint64_t v = 8 * b + t;
+		const bool valid = b < b_end && v >= a.pre_in && v - a.pre_in < a.N;
+		uint64_t v0 = 0, v1 = 0;
+		if (FAST) {
+			const uint64_t i = valid ? v - a.pre_in : 0;
+			const ulonglong2 x = dy_load2(a.in_base + i * a.in_stride + w);
+			v0 = valid ? x.x : 0;
+			v1 = valid ? x.y : 0;
+		} else {
+			// ragged or unaligned tiles: two 8-byte loads at clamped word
+			// indices, out-of-range words replaced by zeros
+			const uint64_t i = valid ? v - a.pre_in : 0;
+			const uint64_t* src = a.in_base + i * a.in_stride;
+			const uint64_t wa = w < a.W ? w : a.W - 1, wb = w + 1 < a.W ? w + 1 : a.W - 1;
+			const uint64_t x0 = dy_load(src + wa), x1 = dy_load(src + wb);
+			v0 = (valid && w < a.W) ? x0 : 0;
+			v1 = (valid && w + 1 < a.W) ? x1 : 0;
+		}
+		d[t][2 * JJ] = make_uint2((uint32_t)v0, (uint32_t)(v0 >> 32));
+		d[t][2 * JJ + 1] = make_uint2((uint32_t)v1, (uint32_t)(v1 >> 32));
+	}
+}
+
+// All products of one word's block: x = the block's 8 input words, transformed
+// to the subset basis here; acc = the 8 outputs in the subset basis.
+__device__ __forceinline__ void dy_word(const uint4* tab, uint2 (&x)[8], uint2 (&acc)[8]) {
+	dy_superset(x);
+#pragma unroll
+	for (int T = 0; T < 8; T++) {
+#pragma unroll
+		for (int h = 0; h < 2; h++) {
+			const uint32_t src = h ? x[T].y : x[T].x;
+			uint32_t ad[8];
+			dy_nib16x8(src, ad);
+			if (T == 7) {
+				// slot a of table Q0 (g'_0): 8-byte reads
+				uint2 v[8];
+#pragma unroll
+				for (int kk = 0; kk < 8; kk++)
+					v[kk] = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(tab) +
+					                                        (4 * 16 + 8 * h + kk) * 256 + ad[kk]);
+				uint2 c = acc[7];
+				c.x = dy_xor3(dy_xor3(dy_xor3(dy_xor3(c.x, v[0].x, v[1].x), v[2].x, v[3].x), v[4].x, v[5].x), v[6].x, v[7].x);
+				c.y = dy_xor3(dy_xor3(dy_xor3(dy_xor3(c.y, v[0].y, v[1].y), v[2].y, v[3].y), v[4].y, v[5].y), v[6].y, v[7].y);
+				acc[7] = c;
+				__builtin_amdgcn_sched_barrier(0);
+			} else {
+#pragma unroll
+				for (int uu = 0; uu < 4; uu++) {
+					if (uu >= kDyNT[T]) break;
+					const int tbl = kDyTab[T][uu];
+					const int ua = kDySA[tbl] | T, ub = kDySB[tbl] | T;
+					uint4 v[8];
+#pragma unroll
+					for (int kk = 0; kk < 8; kk++)
+						v[kk] = *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(tab) +
+						                                        (tbl * 16 + 8 * h + kk) * 256 + ad[kk]);
+					uint2 ca = acc[ua], cb = acc[ub];
+					ca.x = dy_xor3(dy_xor3(dy_xor3(dy_xor3(ca.x, v[0].x, v[1].x), v[2].x, v[3].x), v[4].x, v[5].x), v[6].x, v[7].x);
+					ca.y = dy_xor3(dy_xor3(dy_xor3(dy_xor3(ca.y, v[0].y, v[1].y), v[2].y, v[3].y), v[4].y, v[5].y), v[6].y, v[7].y);
+					cb.x = dy_xor3(dy_xor3(dy_xor3(dy_xor3(cb.x, v[0].z, v[1].z), v[2].z, v[3].z), v[4].z, v[5].z), v[6].z, v[7].z);
+					cb.y = dy_xor3(dy_xor3(dy_xor3(dy_xor3(cb.y, v[0].w, v[1].w), v[2].w, v[3].w), v[4].w, v[5].w), v[6].w, v[7].w);
+					acc[ua] = ca;
+					acc[ub] = cb;
+					// one table's 8 reads in flight at a time: deeper hoisting
+					// runs out of registers at two waves per SIMD
+					__builtin_amdgcn_sched_barrier(0);
+				}
+			}
+		}
+	}
+}
+
+// Block bb's products of word pair JJ, then the pair's inputs of block bb + 1
+// into the same registers.  Every index is a compile-time constant, so d and
+// acc stay in VGPRs.
+template <int WPT, int JJ, bool FAST>
+__device__ __forceinline__ void dy_pair_step(const DyadicArgs& a, const uint4* tab, uint64_t bb,
+                                             uint64_t b_end, uint64_t w0, uint2 (&d)[8][WPT],
+                                             uint2 (&acc)[WPT][8]) {
+	uint2 x0[8], x1[8];
+#pragma unroll
+	for (int t = 0; t < 8; t++) {
+		x0[t] = d[t][2 * JJ];
+		x1[t] = d[t][2 * JJ + 1];
+	}
+	dy_word(tab, x0, acc[2 * JJ]);
+	dy_word(tab, x1, acc[2 * JJ + 1]);
+	dy_load_pair<WPT, JJ, FAST>(a, bb + 1, b_end, w0, d);
+}
+
+// One tile: words [w0, w0 + 256*WPT) of output group q (virtual recovery
+// points 8q..8q+7), input blocks [b_begin, b_end); `tab` is 7 x 256 uint4 of
+// LDS.  out_mode: DY_OUT_DIRECT stores (or XORs when a.accumulate) into the
+// outputs, DY_OUT_PARTIAL into partial row g, DY_OUT_ATOMIC XORs atomically
+// (a job's input range split over tiles of one persistent launch; XOR is
+// order-independent, so the bytes equal the unsplit result).
+enum { DY_OUT_DIRECT = 0, DY_OUT_PARTIAL = 1, DY_OUT_ATOMIC = 2 };
+
+template <int WPT, bool FAST>
+__device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_t q, uint64_t w0,
+                                        uint64_t b_begin, uint64_t b_end, uint64_t g, int out_mode) {
+	static_assert(WPT == 2 || WPT == 4, "WPT");
+	const int tid = threadIdx.x;
+	uint2 acc[WPT][8];
+#pragma unroll
+	for (int j = 0; j < WPT; j++)
+#pragma unroll
+		for (int u = 0; u < 8; u++) acc[j][u] = make_uint2(0, 0);
+
+	// Software pipeline without a second buffer: a word pair's inputs of the
+	// next block are loaded into its registers as soon as this block's
+	// products of the pair are done, so the HBM latency hides behind the other
+	// pairs' lookups and the next table build.
+	uint2 d[8][WPT];
+	uint64_t c0, c1;
+	dy_load_coeffs(a, q, b_begin, b_end, c0, c1);
+	dy_load_pair<WPT, 0, FAST>(a, b_begin, b_end, w0, d);
+	if (WPT == 4) dy_load_pair<WPT, (WPT == 4 ? 1 : 0), FAST>(a, b_begin, b_end, w0, d);
+	for (uint64_t bb = b_begin; bb < b_end; bb++) {
+		// ---- tables: thread (tbl, n) owns entry n of all 16 positions of one
+		// pair table; entry(k, n) = x^4 * entry(k-1, n).  The 16 lanes of a
+		// table write 16 consecutive entries of one row per store.
+		__syncthreads();
+		if (tid < kDyTables * 16) {
+			const int n = tid & 15;
+			const int tbl = tid >> 4;
+			uint64_t e0 = gf64_mul_nibble(c0, (unsigned)n);
+			uint64_t e1 = gf64_mul_nibble(c1, (unsigned)n);
+#pragma unroll
+			for (int k = 0; k < 16; k++) {
+				if (k) {
+					e0 = gf64_mulx4(e0);
+					e1 = gf64_mulx4(e1);
+				}
+				uint4 v;
+				v.x = (uint32_t)e0;
+				v.y = (uint32_t)(e0 >> 32);
+				v.z = (uint32_t)e1;
+				v.w = (uint32_t)(e1 >> 32);
+				tab[(tbl * 16 + k) * 16 + n] = v;
+			}
+		}
+		__syncthreads();
+		dy_load_coeffs(a, q, bb + 1, b_end, c0, c1);
+		dy_pair_step<WPT, 0, FAST>(a, tab, bb, b_end, w0, d, acc);
+		if (WPT == 4) dy_pair_step<WPT, (WPT == 4 ? 1 : 0), FAST>(a, tab, bb, b_end, w0, d, acc);
+	}
+
+	// ---- back to the output basis, then store the real rows of this group
+#pragma unroll
+	for (int j = 0; j < WPT; j++) dy_superset(acc[j]);
+#pragma unroll
+	for (int s = 0; s < 8; s++) {
+		const uint64_t v = 8 * q + s;  // virtual recovery slot
+		if (v < a.pre_out) continue;
+		const uint64_t r = v - a.pre_out;
+		if (r >= a.R) break;
+		uint64_t* dst;
+		bool xr;
+		if (out_mode == DY_OUT_PARTIAL) {
+			dst = a.partial + (g * a.R + r) * a.W;
+			xr = false;
+		} else {
+			dst = a.out_base + r * a.out_stride;
+			xr = a.accumulate != 0;
+		}
+#pragma unroll
+		for (int jj = 0; jj < WPT / 2; jj++) {
+			const uint64_t w = w0 + 2 * tid + 512 * jj;
+			const uint2 p0 = acc[2 * jj][s], p1 = acc[2 * jj + 1][s];
+			uint64_t o0 = ((uint64_t)p0.y << 32) | p0.x;
+			uint64_t o1 = ((uint64_t)p1.y << 32) | p1.x;
+			if (out_mode == DY_OUT_ATOMIC) {
+				if (w < a.W && o0) atomicXor((unsigned long long*)(dst + w), (unsigned long long)o0);
+				if (w + 1 < a.W && o1)
+					atomicXor((unsigned long long*)(dst + w + 1), (unsigned long long)o1);
+			} else if (w + 1 < a.W && ((reinterpret_cast<uintptr_t>(dst + w) & 15u) == 0)) {
+				ulonglong2* qd = reinterpret_cast<ulonglong2*>(dst + w);
+				if (xr) {
+					const ulonglong2 o = *qd;
+					o0 ^= o.x;
+					o1 ^= o.y;
+				}
+				ulonglong2 val;
+				val.x = o0;
+				val.y = o1;
+				*qd = val;
+			} else {
+				if (w < a.W) dst[w] = xr ? (dst[w] ^ o0) : o0;
+				if (w + 1 < a.W) dst[w + 1] = xr ? (dst[w + 1] ^ o1) : o1;
+			}
+		}
+	}
+}
+
+}  // namespace parhip

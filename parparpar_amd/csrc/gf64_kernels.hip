@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "gf64_dev.h"
+#include "gf64_dyadic_dev.h"
 #include "kernels.h"
 
 namespace parhip {
@@ -329,7 +330,27 @@ __device__ __forceinline__ void run_batch_tile(const BatchJob& jb, const BatchTi
 	mac_tab_tile<RT, WPT, IB>(a, tab, tl.w0, tl.r0, tl.i0, tl.i1, 0, tl.mode);
 }
 
-__global__ __launch_bounds__(256) void k_batch_persistent(const BatchJob* __restrict__ jobs,
+template <int WPT>
+__device__ __forceinline__ void run_batch_dy(const BatchJob& jb, const BatchTile& tl, uint4* tab) {
+	DyadicArgs a{};
+	a.in_base = jb.in;
+	a.in_stride = jb.in_stride;
+	a.out_base = jb.out;
+	a.out_stride = jb.out_stride;
+	a.gp = jb.gp;
+	a.gp_ld = jb.nblocks;
+	a.N = jb.N;
+	a.R = jb.R;
+	a.W = jb.W;
+	a.pre_in = jb.pre_in;
+	a.pre_out = jb.pre_out;
+	a.nblocks = jb.nblocks;
+	a.accumulate = 0;
+	dy_tile<WPT, true>(a, tab, tl.r0, tl.w0, tl.i0, tl.i1, 0,
+	                   tl.mode == OUT_ATOMIC ? DY_OUT_ATOMIC : DY_OUT_DIRECT);
+}
+
+__global__ __launch_bounds__(256, 2) void k_batch_persistent(const BatchJob* __restrict__ jobs,
                                                           const BatchTile* __restrict__ tiles,
                                                           uint32_t ntiles, uint32_t* counter) {
 	__shared__ uint4 tab[4096];  // 64 KiB: the largest IB*(RT/2)*256 of the variants
@@ -347,6 +368,8 @@ __global__ __launch_bounds__(256) void k_batch_persistent(const BatchJob* __rest
 			case BATCH_CFG_8_2_4: run_batch_tile<8, 2, 4>(jb, tl, tab); break;
 			case BATCH_CFG_16_2_2: run_batch_tile<16, 2, 2>(jb, tl, tab); break;
 			case BATCH_CFG_4_4_4: run_batch_tile<4, 4, 4>(jb, tl, tab); break;
+			case BATCH_CFG_DY4: run_batch_dy<4>(jb, tl, tab); break;
+			case BATCH_CFG_DY2: run_batch_dy<2>(jb, tl, tab); break;
 			default: break;
 		}
 	}
@@ -371,6 +394,44 @@ __global__ void k_cauchy_batch(const BatchJob* __restrict__ jobs, uint32_t njobs
 		const uint64_t r = e / jb.N, c = e - r * jb.N;
 		const uint64_t d = (jb.fi + c) ^ (jb.fr + r);
 		jb.coeff[e] = gf64_inv(d ? d : 1);
+	}
+}
+
+// XOR-block coefficients of every dyadic job of a batch: 8 threads per
+// (job, q, block) pair, job found by binary search over jobs[j].gp_off; the
+// same values as k_dyadic_coeffs.
+__global__ __launch_bounds__(256) void k_dyadic_coeffs_batch(const BatchJob* __restrict__ jobs,
+                                                             uint32_t njobs, uint64_t total) {
+	__shared__ uint64_t gs[256];
+	const int u = threadIdx.x & 7;
+	for (uint64_t base_pair = (uint64_t)blockIdx.x * 32; base_pair < total;
+	     base_pair += (uint64_t)gridDim.x * 32) {
+		const uint64_t idx = base_pair + (threadIdx.x >> 3);
+		uint64_t c = 0;
+		uint32_t lo = 0;
+		if (idx < total) {
+			uint32_t hi = njobs - 1;
+			while (lo < hi) {
+				const uint32_t mid = (lo + hi + 1) >> 1;
+				if (jobs[mid].gp_off <= idx) lo = mid; else hi = mid - 1;
+			}
+			const BatchJob& jb = jobs[lo];
+			const uint64_t e = idx - jb.gp_off;
+			const uint64_t q = e / jb.nblocks, b = e - q * jb.nblocks;
+			const uint64_t z = (((jb.fi & ~7ull) + 8 * b) ^ ((jb.fr & ~7ull) + 8 * q)) ^ (uint64_t)u;
+			c = z ? gf64_inv(z) : 1u;
+		}
+		gs[threadIdx.x] = c;
+		__syncthreads();
+		uint64_t acc = 0;
+#pragma unroll
+		for (int t = 0; t < 8; t++)
+			if ((t & u) == u) acc ^= gs[(threadIdx.x & ~7) + t];
+		if (idx < total) {
+			const BatchJob& jb = jobs[lo];
+			jb.gp[(idx - jb.gp_off) * 8 + u] = acc;
+		}
+		__syncthreads();
 	}
 }
 
@@ -524,12 +585,19 @@ hipError_t launch_xor_reduce(uint64_t* partial, uint64_t G, uint64_t R, uint64_t
 
 hipError_t launch_batch(const BatchJob* jobs, uint32_t njobs, const BatchTile* tiles,
                         uint32_t ntiles, uint32_t* counter, const uint32_t* zero_ids,
-                        uint32_t nzero, uint64_t coeff_total, int blocks, hipStream_t s) {
+                        uint32_t nzero, uint64_t coeff_total, uint64_t dy_pairs, int blocks,
+                        hipStream_t s) {
 	if (coeff_total) {
 		uint64_t cb = (coeff_total + 255) / 256;
 		if (cb > 65536) cb = 65536;
 		hipLaunchKernelGGL(k_cauchy_batch, dim3((unsigned)cb), dim3(256), 0, s, jobs, njobs,
 		                   coeff_total);
+	}
+	if (dy_pairs) {
+		uint64_t cb = (dy_pairs + 31) / 32;
+		if (cb > 65536) cb = 65536;
+		hipLaunchKernelGGL(k_dyadic_coeffs_batch, dim3((unsigned)cb), dim3(256), 0, s, jobs, njobs,
+		                   dy_pairs);
 	}
 	if (nzero) {
 		hipLaunchKernelGGL(k_zero_outputs, dim3(64, nzero < 1024 ? nzero : 1024), dim3(256), 0, s,

@@ -30,18 +30,26 @@ enum BatchCfg : uint16_t {
 	BATCH_CFG_MFMA = 2,    // GF(2) bit-matrix on MFMA (fp4): job-level launches, no tiles
 	BATCH_CFG_16_2_2 = 3,  // RT=16, WPT=2, IB=2
 	BATCH_CFG_4_4_4 = 4,   // RT=4,  WPT=4, IB=4
-	BATCH_CFG_COUNT = 5
+	BATCH_CFG_DY4 = 5,     // XOR-block (dyadic) tiles, WPT=4 (full 1024-word tiles)
+	BATCH_CFG_DY2 = 6,     // XOR-block (dyadic) tiles, WPT=2 (full 512-word tiles)
+	BATCH_CFG_COUNT = 7
 };
 struct BatchJob {
 	const uint64_t* in;
 	uint64_t in_stride;
 	uint64_t* out;
 	uint64_t out_stride;
-	uint64_t* coeff;      // R x N, built by the batch's Cauchy kernel
+	uint64_t* coeff;      // R x N, built by the batch's Cauchy kernel (table / MFMA jobs)
 	uint64_t N, R, W;
 	uint64_t fi, fr;
-	uint64_t coeff_off;   // first global index of this job's coefficients
+	uint64_t coeff_off;   // first global index of this job's coefficients (0 of them for dyadic jobs)
+	// dyadic jobs: Q x nblocks x 8 XOR-block coefficients (k_dyadic_coeffs layout)
+	uint64_t* gp;
+	uint64_t nblocks, Q, pre_in, pre_out;
+	uint64_t gp_off;      // first global (q, block) pair of this job (Q*nblocks pairs; 0 otherwise)
 };
+// Table tiles: outputs [r0, r0+RT), inputs [i0, i1).  Dyadic tiles: output
+// group r0, input blocks [i0, i1).  Both: words [w0, w0 + 256*WPT).
 struct BatchTile {
 	uint32_t job;
 	uint16_t cfg;         // BatchCfg
@@ -50,9 +58,14 @@ struct BatchTile {
 	uint64_t w0, i0, i1;
 };
 
+// The batch's launch sequence: coefficient matrices (coeff_total words over
+// the table / MFMA jobs), XOR-block coefficients (dy_pairs (q, block) pairs
+// over the dyadic jobs), zeroing of the atomically accumulated jobs' outputs,
+// queue reset, persistent kernel.
 hipError_t launch_batch(const BatchJob* jobs, uint32_t njobs, const BatchTile* tiles,
                         uint32_t ntiles, uint32_t* counter, const uint32_t* zero_ids,
-                        uint32_t nzero, uint64_t coeff_total, int blocks, hipStream_t s);
+                        uint32_t nzero, uint64_t coeff_total, uint64_t dy_pairs, int blocks,
+                        hipStream_t s);
 
 hipError_t launch_cauchy(uint64_t* out, uint64_t N, uint64_t R, uint64_t fi, uint64_t fr,
                          hipStream_t s);

@@ -102,6 +102,13 @@ def _eligible(shape):
     return 64 * r <= w
 
 
+def _dyadic(shape):
+    """XOR-block tiles: R >= 4 and slices of a multiple of 512 words (the
+    tensors here are aligned, at an even stride)."""
+    n, w, r = shape[:3]
+    return r >= 4 and w % 512 == 0
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_mfma_strategy(gpu, oracle, graph):
     """Jobs whose bit-matrix rows fit (64 R <= words per slice) run on the MFMA
@@ -116,7 +123,7 @@ def test_batch_mfma_strategy(gpu, oracle, graph):
     refs = [device.recovery(x, y.shape[0], fi, fr) for (x, y, fi, fr) in jobs]
     b = Batch(jobs, graph=graph, autotune=False, mfma=True)
     for k, s in enumerate(SHAPES):
-        want = P.METHOD_MFMA if _eligible(s) else P.METHOD_TABLE
+        want = P.METHOD_MFMA if _eligible(s) else (P.METHOD_DYADIC if _dyadic(s) else P.METHOD_TABLE)
         assert b.job_method(k) == want, (k, s)
     assert 0 < sum(_eligible(s) for s in SHAPES) < len(SHAPES)
     for rep in range(3):
@@ -172,8 +179,42 @@ def test_autotune_picks_a_strategy_per_shape(gpu):
     for k, (x, y, fi, fr) in enumerate(jobs):
         assert torch.equal(y, device.recovery(x, y.shape[0], fi, fr))
         m = b.job_method(k)
-        assert m in (P.METHOD_TABLE, P.METHOD_MFMA)
+        assert m in (P.METHOD_DYADIC, P.METHOD_MFMA)  # both shapes qualify for XOR-block tiles
         n, w, r = shapes[k][:3]
         assert L.parhip_autotune_lookup_method(n, r, w) == m
-    assert b.job_method(1) == P.METHOD_TABLE    # 64 R > W: MFMA not offered
+    assert b.job_method(1) == P.METHOD_DYADIC    # 64 R > W: MFMA not offered
+    b.close()
+
+
+def test_batch_dyadic_and_table_tiles_mixed(gpu, oracle):
+    """XOR-block tiles (split over atomic tiles, unaligned points, partial
+    output groups) and table tiles (R < 4, slices not a multiple of 512 words)
+    in one persistent launch, graph replayed; every job equals the oracle."""
+    import torch
+    import parparpar_amd as P
+    from parparpar_amd.batch import Batch
+    rng = np.random.default_rng(407)
+    shapes = [(3000, 512, 13, 5, 3), (9, 1024, 4, 0, 9), (2000, 1536, 8, 1, 2001),
+              (100, 700, 9, 0, 100), (40, 512, 2, 0, 40), (513, 2048, 21, 3, 7)]
+    jobs = _jobs(rng, shapes, torch)
+    b = Batch(jobs, graph=True, autotune=False)
+    for k, s in enumerate(shapes):
+        assert b.job_method(k) == (P.METHOD_DYADIC if _dyadic(s) else P.METHOD_TABLE), (k, s)
+    assert b.info()["atomic_jobs"] >= 1
+    for rep in range(3):
+        for (_, y, _, _) in jobs:
+            y.fill_(rep + 3)
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        for k, (x, y, fi, fr) in enumerate(jobs):
+            n, w = x.shape
+            if rep == 0:
+                got = y.cpu().numpy().view(np.uint64)
+                ref = oracle.recovery(x.cpu().numpy().view(np.uint64), y.shape[0], fi, fr)
+                assert np.array_equal(got, ref), (k, shapes[k])
+                if k == 0:
+                    first = y.clone()
+            elif k == 0:
+                assert torch.equal(y, first), rep
     b.close()

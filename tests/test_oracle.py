@@ -319,3 +319,20 @@ def test_par2_odd_tail(oracle):
     c0 = oracle.gf16_coeff(3, 5)
     h = oracle.lib().or_gf16_mul(int(full[6]) | (int(full[7]) << 8), c0)
     assert int(out[1][6]) | (int(out[1][7]) << 8) == t ^ h
+
+
+@pytest.mark.parametrize("n,words,r,threads", [(5, 64, 3, 2), (17, 1000, 9, 4), (40, 16384, 16, 8),
+                                               (3, 31, 2, 1)])
+def test_cpu_baseline_par2_equals_oracle(oracle, n, words, r, threads):
+    """The PAR2 CPU baseline (split-nibble shuffles, threads over outputs; the
+    bench's cpu_baseline for PAR2) gives the oracle's create bytes, including a
+    slice whose words are not a multiple of 32 (scalar tail)."""
+    rng = np.random.default_rng(n * 1000 + words)
+    x = np.frombuffer(rng.bytes(n * words * 2), dtype=np.uint16).reshape(n, words).copy()
+    inums = rng.choice(32768, n, replace=False).astype(np.uint16)
+    exps = np.sort(rng.choice(65535, r, replace=False)).astype(np.uint16)
+    exps[0] = 0  # exponent 0: plain XOR parity
+    got, used = oracle.baseline_par2_create(x, inums, exps, threads)
+    assert used == min(threads, r)
+    ref = oracle.par2_create([row.view(np.uint8) for row in x], inums, exps, words * 2)
+    assert np.array_equal(got.view(np.uint8), ref)
