@@ -558,7 +558,9 @@ int run_contraction_tables(MacArgs a, hipStream_t s) {
 // ------------------------------------------------------------------ dyadic plan
 // Launch shape of the XOR-block contraction: 4 words per thread for slices of
 // at least 1024 words (tables amortised over 1024 words), else 2; split-K rows
-// for two rounds of work-groups per resident slot (2 per CU: 251 / 157 VGPRs).
+// for one round of work-groups per resident slot (2 per CU at 205 / 129
+// VGPRs): at 1024 x 1 MiB, R = 8, 4 rows measured 0.530 + 0.010 ms (kernel +
+// reduce) against 0.535 + 0.015 ms for 8.
 DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev) {
 	DyadicPlan p;
 	p.pre_in = fi & 7;
@@ -576,7 +578,7 @@ DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t
 	}
 	const uint64_t tiles = (W + 256ull * p.wpt - 1) / (256ull * p.wpt);
 	const uint64_t base = tiles * p.Q;
-	const uint64_t target = (uint64_t)device_cus(dev) * 2 * 2;
+	const uint64_t target = (uint64_t)device_cus(dev) * 2;
 	uint64_t groups = base < target ? (target + base - 1) / base : 1;
 	if (t_tune_groups > 0) groups = (uint64_t)t_tune_groups;
 	// at least 2 batches of blocks per row, so the table builds stay amortised
