@@ -256,12 +256,30 @@ int parhip_d_gf64_recovery_with_coeff(const uint64_t *d_inputs, size_t numInputs
                                       size_t blockSize64, const uint64_t *d_coeff,
                                       int accumulate, void *stream);
 
-/* Contraction with Cauchy coefficients for points (firstInput+i, firstRecovery+r);
- * the coefficient matrix is built on the device and cached per shape. */
+/* Contraction with Cauchy coefficients for points (firstInput+i, firstRecovery+r)
+ * (GF64Controller::ComputeRecoveryBlocks, src/par3_engine.cc:1061-1111, on
+ * device buffers); the coefficients are built on the device and cached per
+ * shape (parhip_set_coeff_cache). */
 int parhip_d_gf64_recovery(const uint64_t *d_inputs, size_t numInputs, size_t inputStride64,
                            uint64_t *d_recovery, size_t numRecovery, size_t recoveryStride64,
                            size_t blockSize64, uint64_t firstInput, uint64_t firstRecovery,
                            int accumulate, void *stream);
+
+/* parhip_d_gf64_recovery without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate -- work recorded under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments and bytes; the coefficients and split-K partials live in one
+ * fixed 64 MiB device array per GPU, held until the process ends (the partials
+ * take what the coefficients leave: fewer split-K rows, never an error).
+ * Shapes whose coefficients need more than the array -- (ceil((firstInput%8 +
+ * numInputs)/8) x ceil((firstRecovery%8 + numRecovery)/8)) blocks of 64 B above
+ * 64 MiB -- return PARHIP_ERR_RANGE.  The array keeps the last call's
+ * contents; calls that overlap on one GPU share it and race, so order them on
+ * one stream or with events. */
+int parhip_d_gf64_recovery_static(const uint64_t *d_inputs, size_t numInputs, size_t inputStride64,
+                                  uint64_t *d_recovery, size_t numRecovery, size_t recoveryStride64,
+                                  size_t blockSize64, uint64_t firstInput, uint64_t firstRecovery,
+                                  int accumulate, void *stream);
 
 /* Contraction over scattered slices: d_in_ptrs / d_out_ptrs are device arrays
  * of device pointers (e.g. the known blocks of an archive during repair);

@@ -68,6 +68,7 @@ _SIGS = [
     ("parhip_d_gf64_build_cauchy", _int, [_vp, _sz, _sz, _u64, _u64, _vp]),
     ("parhip_d_gf64_recovery_with_coeff", _int, [_vp, _sz, _sz, _vp, _sz, _sz, _sz, _vp, _int, _vp]),
     ("parhip_d_gf64_recovery", _int, [_vp, _sz, _sz, _vp, _sz, _sz, _sz, _u64, _u64, _int, _vp]),
+    ("parhip_d_gf64_recovery_static", _int, [_vp, _sz, _sz, _vp, _sz, _sz, _sz, _u64, _u64, _int, _vp]),
     ("parhip_d_gf64_recovery_ptrs", _int, [_vp, _sz, _vp, _sz, _sz, _vp, _sz, _int, _vp]),
     ("parhip_gf64_workspace_bytes", _sz, [_sz, _sz, _sz]),
     ("parhip_tune_override", _int, [_int, _int, _int, ctypes.c_int64]),

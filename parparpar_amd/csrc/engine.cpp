@@ -561,7 +561,8 @@ int run_contraction_tables(MacArgs a, hipStream_t s) {
 // for one round of work-groups per resident slot (2 per CU at 205 / 129
 // VGPRs): at 1024 x 1 MiB, R = 8, 4 rows measured 0.530 + 0.010 ms (kernel +
 // reduce) against 0.535 + 0.015 ms for 8.
-DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev) {
+DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev,
+                       size_t max_part_bytes) {
 	DyadicPlan p;
 	p.pre_in = fi & 7;
 	p.pre_out = fr & 7;
@@ -584,6 +585,9 @@ DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t
 	// at least 2 batches of blocks per row, so the table builds stay amortised
 	groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, p.nblocks / (2 * p.ib)));
 	groups = std::min<uint64_t>(groups, 65535);
+	// fewer rows where the caller's partial buffer is smaller (one row needs none)
+	while (groups > 1 && (size_t)xor_reduce_scratch_words(groups, R, W) * sizeof(uint64_t) > max_part_bytes)
+		groups--;
 	uint64_t per = (p.nblocks + groups - 1) / groups;
 	per = (per + p.ib - 1) / p.ib * p.ib;
 	p.groups = (p.nblocks + per - 1) / per;

@@ -57,7 +57,10 @@ struct DyadicPlan {
 	uint64_t groups, blocks_per_group;
 	size_t part_bytes;  // split-K partials + fold tail (0 when groups == 1)
 };
-DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev);
+// max_part_bytes: the largest split-K partial buffer the caller can give
+// (fewer split-K rows below it).
+DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev,
+                       size_t max_part_bytes = SIZE_MAX);
 // Whether `a` (strided slices, this thread's method) takes the dyadic kernel.
 bool dyadic_applies(const MacArgs& a);
 // gp: the coefficients of the launch's first block; gp_ld: blocks per output
