@@ -77,6 +77,7 @@ _SIGS = [
     ("parhip_d_gf16_coeffs", _int, [_vp, _vp, _sz, _vp, _sz, _vp]),
     ("parhip_d_gf16_muladd", _int, [_vp, _sz, _sz, _vp, _sz, _sz, _sz, _vp, _int, _vp]),
     ("parhip_par2_create", _vp, [_int, _sz, ctypes.c_uint]),
+    ("parhip_par2_create_areas", _vp, [_int, _sz, ctypes.c_uint, ctypes.c_uint]),
     ("parhip_par2_destroy", None, [_vp]),
     ("parhip_par2_set_recovery_slices", _int, [_vp, ctypes.c_uint, _vp]),
     ("parhip_par2_set_current_slice_size", _int, [_vp, _sz]),
@@ -90,6 +91,17 @@ _SIGS = [
     ("parhip_par2_get_output", _int, [_vp, ctypes.c_uint, _vp]),
     ("parhip_par2_discard_output", _int, [_vp]),
     ("parhip_par2_input_batch_size", _int, [_vp]),
+    ("parhip_par2_set_slice_size", _int, [_vp, _sz]),
+    ("parhip_par2_wait_for_add", _int, [_vp]),
+    ("parhip_par2_add_input_async", _int, [_vp, _vp, _sz, ctypes.c_uint16, _int, _vp, _vp]),
+    ("parhip_par2_add_input_coeffs_async", _int, [_vp, _vp, _sz, _vp, _int, _vp, _vp]),
+    ("parhip_par2_set_batch_callbacks", _int, [_vp, _vp, _vp, _vp]),
+    ("parhip_par2_get_output_async", _int, [_vp, ctypes.c_uint, _vp, _vp, _vp]),
+    ("parhip_par2_set_min_input_batch_size", _int, [_vp, ctypes.c_uint]),
+    ("parhip_par2_batches_started", _int, [_vp]),
+    ("parhip_par2_staging_areas", _int, [_vp]),
+    ("parhip_par2_free_processing_mem", _int, [_vp]),
+    ("parhip_par2_output_device_ptr", _int, [_vp, ctypes.c_uint, ctypes.POINTER(_vp), ctypes.POINTER(_sz)]),
     ("parhip_batch_create", _vp, [_vp, _sz, ctypes.c_uint]),
     ("parhip_batch_run", _int, [_vp, _vp]),
     ("parhip_batch_info", _int, [_vp, _vp, _vp, _vp]),

@@ -1,4 +1,4 @@
-"""PAR2 recovery over the PAR2ProcHIP backend, in the call shapes of the
+"""PAR2 recovery over the PAR2 backend engine (parhip_par2_*), in the call shapes of the
 reference's GfProc (src/gf.cc:239-735, used by lib/par2.js:112-228).
 
     gp = GfProc(sliceSize)                  # new binding.GfProc(sliceSize, ...)
@@ -15,10 +15,13 @@ import numpy as np
 
 from . import ParhipError, check, lib
 
+# void (*)(void* user, int cksumValid)
+_VALID_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int)
+
 
 class GfProc:
-    def __init__(self, slice_size, device=0, input_batch=0):
-        self._h = lib().parhip_par2_create(device, slice_size, input_batch)
+    def __init__(self, slice_size, device=0, input_batch=0, staging_areas=2):
+        self._h = lib().parhip_par2_create_areas(device, slice_size, input_batch, staging_areas)
         if not self._h:
             raise ParhipError(-4, lib().parhip_last_error().decode())
         self.slice_size = slice_size
@@ -52,6 +55,24 @@ class GfProc:
     def add(self, input_num, buf, flush=False):
         b = np.ascontiguousarray(buf).view(np.uint8)
         check(lib().parhip_par2_add_input(self._h, b.ctypes.data, b.size, input_num, 1 if flush else 0))
+
+    def add_async(self, input_num, buf, flush=False):
+        """Claim a slot and queue the copy; `buf` must stay unchanged until end()
+        (no completion callback)."""
+        b = np.ascontiguousarray(buf).view(np.uint8)
+        check(lib().parhip_par2_add_input_async(self._h, b.ctypes.data, b.size, input_num,
+                                                1 if flush else 0, None, None))
+
+    def batches_started(self):
+        return lib().parhip_par2_batches_started(self._h)
+
+    def output_device_ptr(self, idx):
+        """(device address, row stride in bytes) of recovery slice idx; the row
+        holds the slice's 16-bit words, then its checksum word."""
+        p = ctypes.c_void_p()
+        st = ctypes.c_size_t()
+        check(lib().parhip_par2_output_device_ptr(self._h, idx, ctypes.byref(p), ctypes.byref(st)))
+        return p.value, st.value
 
     def add_coeffs(self, coeffs, buf, flush=False):
         b = np.ascontiguousarray(buf).view(np.uint8)
@@ -90,6 +111,30 @@ class GfProc:
         if rc < 0:
             check(rc)
         return rc == 1, out
+
+    def get_all(self, out):
+        """Every recovery slice into out (R, slice_size) uint8, read back on the
+        engine's copy workers in parallel (asynchronous getOutput); True when
+        every checksum verified."""
+        import threading
+        r = self.num_outputs
+        if out.shape[0] < r or out.shape[1] < self.slice_size or not out.flags["C_CONTIGUOUS"]:
+            raise ValueError("out must be a contiguous (R, slice_size) uint8 array")
+        state = {"left": r, "valid": True}
+        done = threading.Event()
+        lock = threading.Lock()
+
+        def on_valid(user, valid):
+            with lock:
+                state["valid"] = state["valid"] and bool(valid)
+                state["left"] -= 1
+                if state["left"] == 0:
+                    done.set()
+        cb = _VALID_FN(on_valid)
+        for e in range(r):
+            check(lib().parhip_par2_get_output_async(self._h, e, out[e].ctypes.data, cb, None))
+        done.wait()
+        return state["valid"]
 
     def discard_output(self):
         check(lib().parhip_par2_discard_output(self._h))

@@ -185,3 +185,141 @@ def test_large_slices_staged_in_parallel(gpu, oracle, s):
         assert valid, "checksum invalid for output %d" % e
         assert np.array_equal(out, ref[e]), e
     gp.close()
+
+
+def _hip():
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return hip
+
+
+def _patch_words(hip, dptr, edits):
+    """Overwrite 16-bit words of a device row: edits = {word index: new value}."""
+    import ctypes as C
+    for w, v in edits.items():
+        buf = np.array([v], dtype=np.uint16)
+        assert hip.hipMemcpy(dptr + 2 * w, buf.ctypes.data, 2, 1) == 0  # host to device
+
+
+def _read_words(hip, dptr, n):
+    buf = np.zeros(n, dtype=np.uint16)
+    assert hip.hipMemcpy(buf.ctypes.data, dptr, 2 * n, 2) == 0  # device to host
+    return buf
+
+
+@pytest.mark.parametrize("s", [20000, 1 << 20])
+def test_cksum_detects_corruption(gpu, oracle, s):
+    """cksumValid in the negative direction (test/gf16/test-ctrl.cpp:71, :110-112):
+    between endInput and getOutput, one 16-bit word of recovery slice 1 is
+    flipped, two different words of slice 2 are swapped, and the checksum word
+    of slice 3 is changed; those three read back invalid, the untouched slices
+    valid and equal to the oracle."""
+    from parparpar_amd.par2 import GfProc
+    rng = np.random.default_rng(s)
+    nin, nout = 9, 5
+    srcs = [rng.integers(0, 256, size=s, dtype=np.uint8) for _ in range(nin)]
+    inums = rng.choice(32768, nin, replace=False).tolist()
+    exps = [0, 3, 17, 200, 4000]
+    gp = GfProc(s, input_batch=4)
+    gp.set_recovery_slices(exps)
+    for i in range(nin):
+        gp.add(inums[i], srcs[i])
+    gp.end()
+    ref = oracle.par2_create(srcs, inums, exps, s)
+    hip = _hip()
+    words = s // 2
+    d1, _ = gp.output_device_ptr(1)
+    w1 = int(rng.integers(0, words))
+    old = _read_words(hip, d1 + 2 * w1, 1)[0]
+    _patch_words(hip, d1, {w1: old ^ 0x0100})
+    d2, _ = gp.output_device_ptr(2)
+    row = _read_words(hip, d2, words)
+    a = int(rng.integers(0, words))
+    b = next(j for j in range(a + 1, a + words) if row[j % words] != row[a]) % words
+    _patch_words(hip, d2, {a: row[b], b: row[a]})
+    d3, _ = gp.output_device_ptr(3)
+    ck = _read_words(hip, d3 + 2 * words, 1)[0]
+    _patch_words(hip, d3, {words: ck ^ 1})
+    for e in range(nout):
+        valid, out = gp.get(e)
+        if e in (1, 2, 3):
+            assert not valid, "corrupted output %d reported valid" % e
+        else:
+            assert valid and np.array_equal(out, ref[e]), e
+    gp.close()
+
+
+def test_short_input_in_large_slice(gpu, oracle):
+    """Inputs shorter than the slice (one of 100 KiB, one of 3 bytes) in a
+    1 MiB slice: zero-padded staging, valid checksums, the oracle's bytes."""
+    from parparpar_amd.par2 import GfProc
+    rng = np.random.default_rng(77)
+    s = 1 << 20
+    srcs = [rng.integers(0, 256, size=n, dtype=np.uint8) for n in (s, 100 << 10, s, 3)]
+    inums = [5, 900, 31000, 7]
+    exps = [0, 1, 2, 511]
+    gp = GfProc(s, input_batch=2)
+    gp.set_recovery_slices(exps)
+    for i, b in zip(inums, srcs):
+        gp.add(i, b)
+    gp.end()
+    ref = oracle.par2_create(srcs, inums, exps, s)
+    for e in range(len(exps)):
+        valid, out = gp.get(e)
+        assert valid and np.array_equal(out, ref[e]), e
+    gp.close()
+
+
+@pytest.mark.parametrize("areas", [2, 4, 8])
+def test_async_adds_and_passes(gpu, oracle, areas):
+    """Adds that return before their copies (the copy workers stage them), many
+    batches over 2, 4 or 8 staging areas, then a second pass after
+    discardOutput overwrites the first: every output equals the oracle of its
+    pass."""
+    from parparpar_amd.par2 import GfProc
+    rng = np.random.default_rng(78 + areas)
+    s, nout = 6000, 7
+    exps = rng.choice(65535, nout, replace=False).tolist()
+    gp = GfProc(s, input_batch=5, staging_areas=areas)
+    gp.set_recovery_slices(exps)
+    for npass, nin in enumerate((37, 12, 203)):
+        srcs = [rng.integers(0, 256, size=s, dtype=np.uint8) for _ in range(nin)]
+        inums = rng.choice(32768, nin, replace=False).tolist()
+        if npass:
+            gp.discard_output()
+        for i in range(nin):
+            gp.add_async(inums[i], srcs[i])
+        gp.end()
+        ref = oracle.par2_create(srcs, inums, exps, s)
+        for e in range(nout):
+            valid, out = gp.get(e)
+            assert valid and np.array_equal(out, ref[e]), (npass, e)
+    assert gp.batches_started() >= 8 + 3 + 41
+    gp.close()
+
+
+def test_mixed_coefficient_sources_in_one_batch(gpu, oracle):
+    """Inputs with caller-given coefficients and inputs by number share staging
+    areas (a batch with any caller-given column takes its coefficients from the
+    host, the others from the device): outputs equal the oracle's PAR2 create
+    over the same input numbers."""
+    from parparpar_amd.par2 import GfProc, gf16_coeff
+    rng = np.random.default_rng(91)
+    s, nout, nin = 4002, 6, 29
+    exps = rng.choice(65535, nout, replace=False).tolist()
+    srcs = [rng.integers(0, 256, size=s, dtype=np.uint8) for _ in range(nin)]
+    inums = rng.choice(32768, nin, replace=False).tolist()
+    gp = GfProc(s, input_batch=4, staging_areas=3)
+    gp.set_recovery_slices(exps)
+    for i in range(nin):
+        if i % 3 == 1:
+            gp.add_coeffs([gf16_coeff(inums[i], e) for e in exps], srcs[i])
+        else:
+            gp.add_async(inums[i], srcs[i])
+    gp.end()
+    ref = oracle.par2_create(srcs, inums, exps, s)
+    for e in range(nout):
+        valid, out = gp.get(e)
+        assert valid and np.array_equal(out, ref[e]), e
+    gp.close()
