@@ -30,6 +30,9 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE, "all"])
     if os.path.isdir("/root/reference"):
         subprocess.check_call(["make", "-s", "-j8", "-C", HERE, "ref"])
+        # the reference controller over the PAR2ProcHIP backend (links the engine)
+        if os.path.exists(os.path.join(os.path.dirname(HERE), "parparpar_amd", "libparhip.so")):
+            subprocess.check_call(["make", "-s", "-j8", "-C", HERE, "backend"])
 
 
 def lib():
