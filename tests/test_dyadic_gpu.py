@@ -120,25 +120,31 @@ def test_dyadic_streamed_host_entry(gpu, oracle):
 
 @pytest.mark.parametrize("n,w,r", [(1024, 131072, 8), (262144, 512, 8)])
 def test_full_size_dyadic_equals_tables(gpu, oracle, n, w, r):
-    """1 GiB at R=8 in both geometries: the whole output of the dyadic kernel
-    equals that of the direct table kernel, and sampled word columns equal the
-    oracle."""
+    """1 GiB at R=8 in both geometries: the whole outputs of three independent
+    kernels agree -- the dyadic (XOR-block) kernel, the direct nibble-table
+    kernel and the GF(2) bit-matrix on fp4 MFMA -- and sampled word columns
+    equal the oracle."""
     import torch
     from parparpar_amd import device
-    print("full-size dyadic vs tables N=%d W=%d R=%d" % (n, w, r), flush=True)
+    print("full-size dyadic vs tables vs fp4 MFMA N=%d W=%d R=%d" % (n, w, r), flush=True)
     inp = _torch_words(n + w, n, w)
-    _method(gpu, gpu.METHOD_TABLE)
-    tab = device.recovery(inp, r, 0, n)
-    _method(gpu, gpu.METHOD_DYADIC)
-    dy = device.recovery(inp, r, 0, n)
-    _method(gpu, gpu.METHOD_AUTO)
+    try:
+        _method(gpu, gpu.METHOD_TABLE)
+        tab = device.recovery(inp, r, 0, n)
+        _method(gpu, gpu.METHOD_DYADIC)
+        dy = device.recovery(inp, r, 0, n)
+        _method(gpu, gpu.METHOD_MFMA)
+        mf = device.recovery(inp, r, 0, n)
+    finally:
+        _method(gpu, gpu.METHOD_AUTO)
     torch.cuda.synchronize()
     assert torch.equal(tab, dy)
+    assert torch.equal(tab, mf)
     rng = np.random.default_rng(w)
     cols = np.unique(np.concatenate([[0, w - 1], rng.integers(0, w, size=14)])).astype(np.int64)
     tc = torch.as_tensor(cols, device="cuda")
     sample = np.ascontiguousarray(inp[:, tc].cpu().numpy().view(np.uint64))
     ref = oracle.recovery_with_coeff(sample, oracle.cauchy(n, r, 0, n))
     assert np.array_equal(dy[:, tc].cpu().numpy().view(np.uint64), ref)
-    del inp, tab, dy
+    del inp, tab, dy, mf
     torch.cuda.empty_cache()

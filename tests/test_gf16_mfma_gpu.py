@@ -100,3 +100,37 @@ def test_full_size_columns(use_method, oracle, m):
     assert np.array_equal(out[:, tc].cpu().numpy().view(np.uint16), ref)
     del x, out
     torch.cuda.empty_cache()
+
+
+def test_full_size_whole_output_tables_fp4_i8(gpu, oracle):
+    """1 GiB = 32768 x 32 KiB at 512 recovery slices: the whole outputs of the
+    split-table kernel and of the bit-matrix kernel on fp4 and on int8 MFMA
+    agree byte for byte, and sampled columns equal the oracle."""
+    import torch
+    import parparpar_amd as pp
+    from parparpar_amd import device
+    n, words, r = 32768, 16384, 512
+    print("gf16 whole output tables/fp4/i8 N=%d words=%d R=%d" % (n, words, r), flush=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    x = torch.randint(-32768, 32767, (n, words), dtype=torch.int16, device="cuda", generator=g)
+    inums = np.arange(n, dtype=np.uint16)
+    exps = np.arange(r, dtype=np.uint16)
+    co = device.gf16_coeffs(inums, exps)
+    outs = {}
+    try:
+        for name, m in (("tab", pp.METHOD_TABLE), ("fp4", pp.METHOD_MFMA), ("i8", pp.METHOD_MFMA_I8)):
+            pp.set_method(m)
+            outs[name] = device.gf16_muladd(x, co)
+    finally:
+        pp.set_method(pp.METHOD_AUTO)
+    torch.cuda.synchronize()
+    assert torch.equal(outs["tab"], outs["fp4"])
+    assert torch.equal(outs["tab"], outs["i8"])
+    cols = np.array([0, 7, 8191, words - 1], dtype=np.int64)
+    tc = torch.as_tensor(cols, device="cuda")
+    samp = np.ascontiguousarray(x[:, tc].cpu().numpy().view(np.uint8))
+    ref = oracle.par2_columns(samp, 2 * len(cols), inums, exps, np.arange(len(cols), dtype=np.uint64))
+    assert np.array_equal(outs["tab"][:, tc].cpu().numpy().view(np.uint16), ref)
+    del x, outs
+    torch.cuda.empty_cache()
