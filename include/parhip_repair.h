@@ -25,7 +25,8 @@ extern "C" {
 #endif
 
 /* Rebuild the k input slices `missing` (host array of input indices) of an
- * archive: data slice i at d_data + i*dataStride64 (lost slots are not read),
+ * archive: data slice i at d_data + i*dataStride64 (the lost slots' memory may
+ * hold anything: it may be read but never contributes),
  * recovery slice r at d_recovery + r*recoveryStride64 with Cauchy points
  * (firstInput + i, firstRecovery + r).  recoveryRows (host, k indices) selects
  * the recovery slices used, NULL = the first k (lib/par3gen.js:3892-3901).
@@ -42,9 +43,13 @@ size_t parhip_gf64_repair_workspace_bytes(size_t numInputs, size_t blockSize64, 
 
 typedef struct parhip_repair_server parhip_repair_server;
 
-/* hbmBudgetBytes bounds the device memory of requests in flight (workspace +
- * outputs not yet released); maxInflight bounds concurrent requests;
- * numStreams (0 = maxInflight) streams. */
+/* hbmBudgetBytes bounds the server's device memory: the workspace of requests
+ * in flight, outputs not yet released and cached inverses.  A request waits
+ * while it does not fit (after evicting unused cached inverses), whether or not
+ * others are in flight: a caller that holds outputs unreleased while it waits
+ * for a later request can stall it.  A request that can never fit is refused
+ * at submit.  maxInflight bounds concurrent requests; numStreams (0 =
+ * maxInflight) streams. */
 parhip_repair_server *parhip_repair_server_create(size_t hbmBudgetBytes, unsigned maxInflight,
                                                   unsigned numStreams);
 /* Register a device-resident archive; returns its id (>= 0) or an error. */
@@ -64,6 +69,12 @@ int parhip_repair_result(parhip_repair_server *s, int64_t id, uint64_t **d_out, 
 int parhip_repair_release(parhip_repair_server *s, int64_t id);
 int parhip_repair_server_stats(parhip_repair_server *s, uint64_t *completed, double *p50_ms,
                                double *p99_ms, unsigned *peak_inflight, uint64_t *peak_bytes);
+/* The resident inverse cache: inverted k x k matrices kept in HBM, keyed by
+ * (archive, missing slices in request order), charged to the HBM budget and
+ * evicted least recently used first when a request needs the memory.  hits /
+ * misses count admitted requests; entries / bytes are the current contents. */
+int parhip_repair_server_cache_stats(parhip_repair_server *s, uint64_t *hits, uint64_t *misses,
+                                     uint64_t *entries, uint64_t *bytes, uint64_t *evictions);
 /* Drains queued work, then frees everything. */
 void parhip_repair_server_destroy(parhip_repair_server *s);
 

@@ -122,6 +122,7 @@ _SIGS = [
     ("parhip_repair_result", _int, [_vp, ctypes.c_int64, ctypes.POINTER(_vp), ctypes.POINTER(_sz)]),
     ("parhip_repair_release", _int, [_vp, ctypes.c_int64]),
     ("parhip_repair_server_stats", _int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("parhip_repair_server_cache_stats", _int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("parhip_repair_server_destroy", None, [_vp]),
     ("parhip_prof_enable", _int, [_int]),
     ("parhip_prof_reset", _int, []),

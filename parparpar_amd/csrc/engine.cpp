@@ -596,18 +596,27 @@ DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t
 	return p;
 }
 
-bool dyadic_applies(const MacArgs& a) {
-	if (t_method != PARHIP_METHOD_AUTO && t_method != PARHIP_METHOD_DYADIC) return false;
+bool dyadic_shape_ok(uint64_t R) {
 	// 27 products per 8 x 8 block against 8 per input and output: the direct
 	// tables cost less below 4 outputs
-	if (a.in_ptrs || a.out_ptrs || a.R < 4) return false;
-	const uint64_t Q = ((a.R + 7) / 8) + 1;
+	if (R < 4) return false;
+	const uint64_t Q = ((R + 7) / 8) + 1;
 	return Q <= 65535;
 }
 
+bool dyadic_applies_shape(uint64_t R) {
+	if (t_method != PARHIP_METHOD_AUTO && t_method != PARHIP_METHOD_DYADIC) return false;
+	return dyadic_shape_ok(R);
+}
+
+bool dyadic_applies(const MacArgs& a) {
+	return !a.in_ptrs && !a.out_ptrs && dyadic_applies_shape(a.R);
+}
+
 hipError_t enqueue_dyadic(const MacArgs& a, const DyadicPlan& p, const uint64_t* gp,
-                          uint64_t gp_ld, void* ws, hipStream_t s) {
+                          uint64_t gp_ld, void* ws, hipStream_t s, const uint32_t* lost) {
 	DyadicArgs d{};
+	d.lost = lost;
 	d.in_base = a.in_base;
 	d.in_stride = a.in_stride;
 	d.out_base = a.out_base;

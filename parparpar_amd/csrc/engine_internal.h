@@ -63,10 +63,15 @@ DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t
                        size_t max_part_bytes = SIZE_MAX);
 // Whether `a` (strided slices, this thread's method) takes the dyadic kernel.
 bool dyadic_applies(const MacArgs& a);
+// Whether R outputs take it under this thread's method (any input layout);
+// dyadic_shape_ok: under the automatic choice.
+bool dyadic_applies_shape(uint64_t R);
+bool dyadic_shape_ok(uint64_t R);
 // gp: the coefficients of the launch's first block; gp_ld: blocks per output
-// group row of the coefficient array.
+// group row of the coefficient array; lost: optional per-block masks of inputs
+// that count as zero (DyadicArgs::lost).
 hipError_t enqueue_dyadic(const MacArgs& a, const DyadicPlan& p, const uint64_t* gp,
-                          uint64_t gp_ld, void* ws, hipStream_t s);
+                          uint64_t gp_ld, void* ws, hipStream_t s, const uint32_t* lost = nullptr);
 // The plain Cauchy contraction of a call with n_total inputs at points
 // (fi + i, fr + r), run over its inputs [first, first + a.N): the dyadic kernel
 // where it applies, else the cached coefficient matrix and run_contraction.

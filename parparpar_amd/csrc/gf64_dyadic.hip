@@ -11,8 +11,8 @@ namespace parhip {
 
 // Work-group: word tile blockIdx.x, split-K row blockIdx.y, output group
 // blockIdx.z.  FAST: every tile is full and 16-byte aligned (checked by the
-// launcher).
-template <int WPT, bool FAST>
+// launcher).  MASK: inputs flagged in a.lost count as zero.
+template <int WPT, bool FAST, bool MASK>
 __global__ __launch_bounds__(256, 2) void k_mac_dyadic(DyadicArgs a) {
 	// tab[(tbl*16 + k)*16 + n] = (g'_SA * n x^{4k}, g'_SB * n x^{4k})
 	__shared__ uint4 tab[kDyTables * 256];
@@ -20,8 +20,8 @@ __global__ __launch_bounds__(256, 2) void k_mac_dyadic(DyadicArgs a) {
 	const uint64_t b_begin = g * a.blocks_per_group;
 	uint64_t b_end = b_begin + a.blocks_per_group;
 	if (b_end > a.nblocks) b_end = a.nblocks;
-	dy_tile<WPT, FAST>(a, tab, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end, g,
-	                   a.partial ? DY_OUT_PARTIAL : DY_OUT_DIRECT);
+	dy_tile<WPT, FAST, MASK>(a, tab, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end, g,
+	                         a.partial ? DY_OUT_PARTIAL : DY_OUT_DIRECT);
 }
 
 // Transformed coefficients of every (output group q, input block b):
@@ -75,16 +75,18 @@ hipError_t launch_mac_dyadic(const DyadicArgs& a, uint64_t groups, int wpt, int 
 	dim3 grid((unsigned)tiles, (unsigned)groups, (unsigned)Q);
 	const bool fast = ((reinterpret_cast<uintptr_t>(a.in_base) & 15u) == 0) && (a.in_stride & 1) == 0 &&
 	                  a.W % (256ull * wpt) == 0;
-	if (wpt == 4 && ib == 1 && fast)
-		hipLaunchKernelGGL((k_mac_dyadic<4, true>), grid, dim3(256), 0, s, a);
-	else if (wpt == 4 && ib == 1)
-		hipLaunchKernelGGL((k_mac_dyadic<4, false>), grid, dim3(256), 0, s, a);
-	else if (wpt == 2 && ib == 1 && fast)
-		hipLaunchKernelGGL((k_mac_dyadic<2, true>), grid, dim3(256), 0, s, a);
-	else if (wpt == 2 && ib == 1)
-		hipLaunchKernelGGL((k_mac_dyadic<2, false>), grid, dim3(256), 0, s, a);
-	else
-		return hipErrorInvalidValue;
+	if (ib != 1 || (wpt != 4 && wpt != 2)) return hipErrorInvalidValue;
+	const int v = (wpt == 4 ? 4 : 0) | (fast ? 2 : 0) | (a.lost ? 1 : 0);
+	switch (v) {
+		case 6: hipLaunchKernelGGL((k_mac_dyadic<4, true, false>), grid, dim3(256), 0, s, a); break;
+		case 7: hipLaunchKernelGGL((k_mac_dyadic<4, true, true>), grid, dim3(256), 0, s, a); break;
+		case 4: hipLaunchKernelGGL((k_mac_dyadic<4, false, false>), grid, dim3(256), 0, s, a); break;
+		case 5: hipLaunchKernelGGL((k_mac_dyadic<4, false, true>), grid, dim3(256), 0, s, a); break;
+		case 2: hipLaunchKernelGGL((k_mac_dyadic<2, true, false>), grid, dim3(256), 0, s, a); break;
+		case 3: hipLaunchKernelGGL((k_mac_dyadic<2, true, true>), grid, dim3(256), 0, s, a); break;
+		case 0: hipLaunchKernelGGL((k_mac_dyadic<2, false, false>), grid, dim3(256), 0, s, a); break;
+		default: hipLaunchKernelGGL((k_mac_dyadic<2, false, true>), grid, dim3(256), 0, s, a); break;
+	}
 	return hipGetLastError();
 }
 

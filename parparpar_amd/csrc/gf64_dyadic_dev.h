@@ -123,14 +123,27 @@ __device__ __forceinline__ void dy_load_coeffs(const DyadicArgs& a, uint64_t q, 
 	}
 }
 
+// The block's mask of inputs that count as zero (MASK launches only; the
+// block index is the same for the whole work-group, so this is one scalar).
+template <bool MASK>
+__device__ __forceinline__ uint32_t dy_lost(const DyadicArgs& a, uint64_t b, uint64_t b_end) {
+	if (!MASK) return 0u;
+	// a scalar load (constant address space): the mask is read-only here
+	typedef const __attribute__((address_space(4))) uint32_t cu32;
+	return ((cu32*)(a.lost))[b < b_end ? b : 0];  // (past the end: unused)
+}
+
 // This thread's word pair JJ (words w0 + 2*tid + 512*JJ, +1) of the 8 inputs of
 // block b: zeros for virtual inputs, past the group's blocks or past the slice
-// end.  Indices are clamped into range and out-of-range words replaced by zeros
-// with selects, so the loads are straight-line code; FAST (full, 16-byte
-// aligned tiles) loads each pair with one 16-byte load.
+// end, and for inputs flagged in `lost` (they are loaded like the others and
+// ANDed to zero: a lost slice's memory may hold anything).  Indices are clamped
+// into range and out-of-range words replaced by zeros with selects, so the
+// loads are straight-line code (a per-input condition on the load itself makes
+// the compiler spill the whole pipeline); FAST (full, 16-byte aligned tiles)
+// loads each pair with one 16-byte load.
 template <int WPT, int JJ, bool FAST>
 __device__ __forceinline__ void dy_load_pair(const DyadicArgs& a, uint64_t b, uint64_t b_end,
-                                             uint64_t w0, uint2 (&d)[8][WPT]) {
+                                             uint64_t w0, uint32_t lost, uint2 (&d)[8][WPT]) {
 	const int tid = threadIdx.x;
 	const uint64_t w = w0 + 2 * tid + 512 * JJ;
 #pragma unroll
@@ -138,12 +151,14 @@ __device__ __forceinline__ void dy_load_pair(const DyadicArgs& a, uint64_t b, ui
 		// real input of virtual slot t of block b
 		const uint64_t v = 8 * b + t;
 		const bool valid = b < b_end && v >= a.pre_in && v - a.pre_in < a.N;
+		// all ones unless the input counts as zero (a scalar per block)
+		const uint32_t keep = 0u - ((~lost >> t) & 1u);
 		uint64_t v0 = 0, v1 = 0;
 		if (FAST) {
 			const uint64_t i = valid ? v - a.pre_in : 0;
 			const ulonglong2 x = dy_load2(a.in_base + i * a.in_stride + w);
-			v0 = valid ? x.x : 0;
-			v1 = valid ? x.y : 0;
+			v0 = (valid ? x.x : 0) & ((uint64_t)keep << 32 | keep);
+			v1 = (valid ? x.y : 0) & ((uint64_t)keep << 32 | keep);
 		} else {
 			// ragged or unaligned tiles: two 8-byte loads at clamped word
 			// indices, out-of-range words replaced by zeros
@@ -151,8 +166,8 @@ __device__ __forceinline__ void dy_load_pair(const DyadicArgs& a, uint64_t b, ui
 			const uint64_t* src = a.in_base + i * a.in_stride;
 			const uint64_t wa = w < a.W ? w : a.W - 1, wb = w + 1 < a.W ? w + 1 : a.W - 1;
 			const uint64_t x0 = dy_load(src + wa), x1 = dy_load(src + wb);
-			v0 = (valid && w < a.W) ? x0 : 0;
-			v1 = (valid && w + 1 < a.W) ? x1 : 0;
+			v0 = ((valid && w < a.W) ? x0 : 0) & ((uint64_t)keep << 32 | keep);
+			v1 = ((valid && w + 1 < a.W) ? x1 : 0) & ((uint64_t)keep << 32 | keep);
 		}
 		d[t][2 * JJ] = make_uint2((uint32_t)v0, (uint32_t)(v0 >> 32));
 		d[t][2 * JJ + 1] = make_uint2((uint32_t)v1, (uint32_t)(v1 >> 32));
@@ -214,8 +229,8 @@ __device__ __forceinline__ void dy_word(const uint4* tab, uint2 (&x)[8], uint2 (
 // acc stay in VGPRs.
 template <int WPT, int JJ, bool FAST>
 __device__ __forceinline__ void dy_pair_step(const DyadicArgs& a, const uint4* tab, uint64_t bb,
-                                             uint64_t b_end, uint64_t w0, uint2 (&d)[8][WPT],
-                                             uint2 (&acc)[WPT][8]) {
+                                             uint64_t b_end, uint64_t w0, uint32_t lost_next,
+                                             uint2 (&d)[8][WPT], uint2 (&acc)[WPT][8]) {
 	uint2 x0[8], x1[8];
 #pragma unroll
 	for (int t = 0; t < 8; t++) {
@@ -224,7 +239,7 @@ __device__ __forceinline__ void dy_pair_step(const DyadicArgs& a, const uint4* t
 	}
 	dy_word(tab, x0, acc[2 * JJ]);
 	dy_word(tab, x1, acc[2 * JJ + 1]);
-	dy_load_pair<WPT, JJ, FAST>(a, bb + 1, b_end, w0, d);
+	dy_load_pair<WPT, JJ, FAST>(a, bb + 1, b_end, w0, lost_next, d);
 }
 
 // One tile: words [w0, w0 + 256*WPT) of output group q (virtual recovery
@@ -235,7 +250,7 @@ __device__ __forceinline__ void dy_pair_step(const DyadicArgs& a, const uint4* t
 // order-independent, so the bytes equal the unsplit result).
 enum { DY_OUT_DIRECT = 0, DY_OUT_PARTIAL = 1, DY_OUT_ATOMIC = 2 };
 
-template <int WPT, bool FAST>
+template <int WPT, bool FAST, bool MASK = false>
 __device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_t q, uint64_t w0,
                                         uint64_t b_begin, uint64_t b_end, uint64_t g, int out_mode) {
 	static_assert(WPT == 2 || WPT == 4, "WPT");
@@ -253,8 +268,9 @@ __device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_
 	uint2 d[8][WPT];
 	uint64_t c0, c1;
 	dy_load_coeffs(a, q, b_begin, b_end, c0, c1);
-	dy_load_pair<WPT, 0, FAST>(a, b_begin, b_end, w0, d);
-	if (WPT == 4) dy_load_pair<WPT, (WPT == 4 ? 1 : 0), FAST>(a, b_begin, b_end, w0, d);
+	uint32_t lost = dy_lost<MASK>(a, b_begin, b_end);
+	dy_load_pair<WPT, 0, FAST>(a, b_begin, b_end, w0, lost, d);
+	if (WPT == 4) dy_load_pair<WPT, (WPT == 4 ? 1 : 0), FAST>(a, b_begin, b_end, w0, lost, d);
 	for (uint64_t bb = b_begin; bb < b_end; bb++) {
 		// ---- tables: thread (tbl, n) owns entry n of all 16 positions of one
 		// pair table; entry(k, n) = x^4 * entry(k-1, n).  The 16 lanes of a
@@ -281,8 +297,9 @@ __device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_
 		}
 		__syncthreads();
 		dy_load_coeffs(a, q, bb + 1, b_end, c0, c1);
-		dy_pair_step<WPT, 0, FAST>(a, tab, bb, b_end, w0, d, acc);
-		if (WPT == 4) dy_pair_step<WPT, (WPT == 4 ? 1 : 0), FAST>(a, tab, bb, b_end, w0, d, acc);
+		lost = dy_lost<MASK>(a, bb + 1, b_end);
+		dy_pair_step<WPT, 0, FAST>(a, tab, bb, b_end, w0, lost, d, acc);
+		if (WPT == 4) dy_pair_step<WPT, (WPT == 4 ? 1 : 0), FAST>(a, tab, bb, b_end, w0, lost, d, acc);
 	}
 
 	// ---- back to the output basis, then store the real rows of this group

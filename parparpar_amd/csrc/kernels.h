@@ -87,6 +87,9 @@ struct DyadicArgs {
 	uint64_t blocks_per_group; // split-K: blocks per work-group row
 	uint64_t* partial;         // if non-null: G x R x W partials, reduced afterwards
 	int accumulate;
+	// if non-null: bit t of lost[b] set = the input at virtual slot 8b+t
+	// counts as zero whatever its memory holds (the known inputs of a repair)
+	const uint32_t* lost;
 };
 hipError_t launch_dyadic_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uint64_t xbase,
                                 uint64_t ybase, hipStream_t s);

@@ -15,7 +15,8 @@ def _s(stream):
 def repair(data, recovery, missing, first_input=0, first_recovery=None, rows=None, out=None,
            stream=None):
     """Rebuild the input slices `missing` of an archive.
-    data: (N, W) int64 device tensor (lost rows are not read); recovery: (R, W)."""
+    data: (N, W) int64 device tensor (lost rows may hold anything: they never
+    contribute); recovery: (R, W)."""
     n, w = data.shape
     r = recovery.shape[0]
     fr = n if first_recovery is None else first_recovery
@@ -78,6 +79,12 @@ class RepairServer:
                                                ctypes.byref(p99), ctypes.byref(pk), ctypes.byref(pb)))
         return {"completed": c.value, "p50_ms": p50.value, "p99_ms": p99.value,
                 "peak_inflight": pk.value, "peak_bytes": pb.value}
+
+    def cache_stats(self):
+        """The resident inverse cache: hits, misses, entries, bytes, evictions."""
+        v = [ctypes.c_uint64() for _ in range(5)]
+        check(lib().parhip_repair_server_cache_stats(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("hits", "misses", "entries", "bytes", "evictions"), (x.value for x in v)))
 
     def close(self):
         if self._h:

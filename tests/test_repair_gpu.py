@@ -26,7 +26,7 @@ def test_repair_round_trip(gpu, n, w, r, k):
     rng = np.random.default_rng(n + k)
     missing = np.sort(rng.choice(n, k, replace=False))
     damaged = data.clone()
-    damaged[torch.as_tensor(missing, device="cuda")] = 0x0BAD  # lost slots are never read
+    damaged[torch.as_tensor(missing, device="cuda")] = 0x0BAD  # lost slots never contribute
     out = repair.repair(damaged, rec, missing)
     torch.cuda.synchronize()
     assert torch.equal(out, data[torch.as_tensor(missing, device="cuda")])
@@ -223,4 +223,131 @@ def test_server_colliding_points(gpu):
             assert torch.equal(got, alone[i])
             assert torch.equal(got, data[torch.as_tensor(miss, device="cuda")])
         srv.release(rid)
+    srv.close()
+
+
+# The two RHS routes: the create's XOR-block contraction over the whole archive
+# with the lost inputs masked to zero (consecutive rows, k >= 4) and the
+# explicit-coefficient tables (forced with the table method).  Points start
+# off a multiple of 8, so virtual slots precede the first input and the first
+# recovery point; lost inputs fall in one block of 8, across blocks and at the
+# ends; the lost slots hold junk.
+ROUTE_CASES = [
+    # (n, w, r, fi, fr, missing, rows)
+    (300, 513, 40, 0x10003, 0x10003 + 305, list(range(0, 9)) + [299], None),
+    (300, 512, 40, 5, 311, [3, 150, 151, 152, 153, 154, 155, 156, 157, 158, 299], list(range(3, 14))),
+    (64, 2048, 16, 0, 64, [7, 8, 15, 16, 63], None),
+    (1000, 96, 64, 1, 1001, list(range(0, 1000, 17))[:58], None),
+    (40, 1024, 8, 7, 47, [0, 1, 2, 3, 4, 5, 6, 39], None),
+]
+
+
+@pytest.mark.parametrize("case", ROUTE_CASES, ids=["%dx%d-r%d-fi%d-k%d" % (c[0], c[1], c[2], c[3], len(c[5]))
+                                                    for c in ROUTE_CASES])
+def test_repair_routes_agree(gpu, case):
+    import torch
+    import parparpar_amd as P
+    from parparpar_amd import device, repair
+    n, w, r, fi, fr, missing, rows = case
+    g = torch.Generator(device="cuda")
+    g.manual_seed(n * 7 + len(missing))
+    data = torch.randint(-2 ** 63, 2 ** 63 - 1, (n, w), dtype=torch.int64, device="cuda", generator=g)
+    rec = device.recovery(data, r, fi, fr)
+    damaged = data.clone()
+    tm = torch.as_tensor(missing, device="cuda")
+    damaged[tm] = torch.randint(-2 ** 63, 2 ** 63 - 1, (len(missing), w), dtype=torch.int64,
+                                device="cuda", generator=g)
+    outs = []
+    try:
+        for m in (P.METHOD_AUTO, P.METHOD_TABLE):
+            P.set_method(m)
+            outs.append(repair.repair(damaged, rec, missing, first_input=fi, first_recovery=fr, rows=rows))
+    finally:
+        P.set_method(P.METHOD_AUTO)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], data[tm])
+    assert torch.equal(outs[1], data[tm])
+
+
+def test_server_inverse_cache_hits_and_eviction(gpu):
+    """The same lost set again takes the cached inverse (hits); a budget that
+    holds one request and about one inverse evicts the least recently used
+    entry for the next; every result equals the original slices and the
+    server's memory never exceeds the budget."""
+    import torch
+    from parparpar_amd import device, lib
+    from parparpar_amd.repair import RepairServer, copy_result
+    n, w, r = 500, 512, 64
+    d, rc = _archive(torch, device, n, w, r, 77)
+    rng = np.random.default_rng(77)
+    sets = [np.sort(rng.choice(n, 64, replace=False)) for _ in range(3)]
+    srv = RepairServer(8 << 30, max_inflight=2)
+    srv.add_archive(d, rc)
+    for rep in range(3):
+        for miss in sets:
+            rid = srv.submit(0, miss)
+            srv.wait(rid)
+            got = copy_result(srv, rid, w)
+            srv.release(rid)
+            assert torch.equal(got, d[torch.as_tensor(miss, device="cuda")]), rep
+    st = srv.cache_stats()
+    assert st["misses"] == 3 and st["hits"] == 6 and st["entries"] == 3, st
+    assert st["bytes"] == 3 * 64 * 64 * 8
+    srv.close()
+    # the server's peak for one request (its charge and its new inverse) plus
+    # half an inverse: room for one cached inverse besides a request, not two
+    srv = RepairServer(8 << 30, max_inflight=1)
+    srv.add_archive(d, rc)
+    rid = srv.submit(0, sets[0])
+    srv.wait(rid)
+    srv.release(rid)
+    budget = srv.stats()["peak_bytes"] + 64 * 64 * 8 // 2
+    srv.close()
+    srv = RepairServer(budget, max_inflight=1)
+    srv.add_archive(d, rc)
+    for miss in sets + sets:
+        rid = srv.submit(0, miss)
+        srv.wait(rid)
+        got = copy_result(srv, rid, w)
+        srv.release(rid)
+        assert torch.equal(got, d[torch.as_tensor(miss, device="cuda")])
+    st = srv.cache_stats()
+    assert st["entries"] <= 1 and st["evictions"] >= 4, st
+    assert srv.stats()["peak_bytes"] <= budget
+    srv.close()
+
+
+def test_server_waits_for_release_with_nothing_in_flight(gpu):
+    """Admission honours the budget even when nothing is in flight: a request
+    that fits only once an earlier output is released waits for the release,
+    then completes correctly."""
+    import threading
+    import torch
+    from parparpar_amd import device, lib
+    from parparpar_amd.repair import RepairServer, copy_result
+    n, w, r, k = 200, 4096, 16, 16
+    d, rc = _archive(torch, device, n, w, r, 12)
+    # one request's peak (its charge and its cached inverse) plus half an
+    # output: a second request fits only once the first output is released
+    srv = RepairServer(8 << 30, max_inflight=4)
+    srv.add_archive(d, rc)
+    rid = srv.submit(0, list(range(2 * k, 3 * k)))
+    srv.wait(rid)
+    srv.release(rid)
+    budget = srv.stats()["peak_bytes"] + k * w * 8 // 2
+    srv.close()
+    srv = RepairServer(budget, max_inflight=4)
+    srv.add_archive(d, rc)
+    a = srv.submit(0, list(range(k)))
+    srv.wait(a)                       # done; its output stays charged
+    b = srv.submit(0, list(range(k, 2 * k)))
+    done = threading.Event()
+    threading.Thread(target=lambda: (srv.wait(b), done.set())).start()
+    assert not done.wait(0.5), "admitted beyond the budget"
+    got_a = copy_result(srv, a, w)
+    srv.release(a)
+    assert done.wait(30), "not admitted after the release"
+    got_b = copy_result(srv, b, w)
+    srv.release(b)
+    assert torch.equal(got_a, d[:k]) and torch.equal(got_b, d[k:2 * k])
     srv.close()
