@@ -16,7 +16,15 @@
  *                       coprime to 3,5,17,257 in increasing order.
  *   or_gf16_coeff     - exponent 0 gives 1 for every input (plain XOR parity,
  *                       gf16/controller_cpu.cpp:495-513); coeff(i,e) equals
- *                       exp(log_i)^e by repeated multiplication.
+ *                       exp(log_i)^e by repeated multiplication; equals the
+ *                       reference's gfmat_coeff compiled in oracle/_ref.
+ *   or_par2_create    - restates the controller test's reference computation
+ *                       (test/gf16/test-ctrl.cpp:185-202) over the pinned
+ *                       multiply and coefficients; its odd-tail rule is
+ *                       checked in test_par2_odd_tail.
+ *   or_par2_recovery_columns
+ *                     - equals or_par2_create on every column of small cases
+ *                       (test_par2_columns_equal_create).
  */
 #include "oracle.h"
 #include <string.h>

@@ -17,7 +17,8 @@
  *                      and the KAT 0x7ffffffffffffffa (test/par3-native-fidelity.js:240-244);
  *                      byte-equal to the reference engine built in oracle/_ref.
  *   region ops       - linearity and the reference's scalar semantics, byte-equal
- *                      to oracle/_ref's gf64_region_*_scalar functions.
+ *                      to oracle/_ref's gf64_region_*_scalar functions
+ *                      (test_region_ops_equal_reference_scalar).
  *   solve            - repaired blocks equal the dropped originals (round trip).
  */
 #include "oracle.h"

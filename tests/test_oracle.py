@@ -336,3 +336,83 @@ def test_cpu_baseline_par2_equals_oracle(oracle, n, words, r, threads):
     assert used == min(threads, r)
     ref = oracle.par2_create([row.view(np.uint8) for row in x], inums, exps, words * 2)
     assert np.array_equal(got.view(np.uint8), ref)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref",
+                                                    "libref_par3.so")),
+                    reason="reference engine not built here (oracle/_ref)")
+def test_region_ops_equal_reference_scalar(oracle):
+    """The six region operations against the reference's own scalar versions
+    (gf64/gf64_region_scalar.c, compiled into oracle/_ref), byte for byte, on
+    random words, coefficient cycling (n_coeff < len) and several blocks."""
+    import ctypes
+    L, R = oracle.lib(), oracle.ref()
+    P = oracle.p64
+    rng = np.random.default_rng(0x5CA1)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+
+    def rnd(*shape):
+        return rng.integers(0, 2 ** 64, size=shape, dtype=np.uint64)
+
+    def ptrs(rows):
+        return (u64p * len(rows))(*[P(r) for r in rows])
+
+    ln = 61
+    x = rnd(ln)
+    co = rnd(7)
+    for fo, fr_ in (("or_gf64_region_mul_arr", "gf64_region_mul_scalar_arr"),
+                    ("or_gf64_region_muladd_arr", "gf64_region_muladd_scalar_arr")):
+        for nc in (1, 3, 7):
+            base = rnd(ln)
+            a, b = base.copy(), base.copy()
+            getattr(L, fo)(P(a), P(x), P(co), ln, nc)
+            getattr(R, fr_)(P(b), P(x), P(co), ctypes.c_size_t(ln), ctypes.c_size_t(nc))
+            assert np.array_equal(a, b), (fo, nc)
+    a, b = rnd(ln), np.zeros(ln, dtype=np.uint64)
+    L.or_gf64_region_mul(P(a), P(x), ln, int(co[0]))
+    R.gf64_region_mul_scalar(P(b), P(x), ctypes.c_size_t(ln), ctypes.c_uint64(int(co[0])))
+    assert np.array_equal(a, b)
+    # coupled: G input blocks, one coefficient each
+    G, K = 5, 4
+    blocks = [rnd(ln) for _ in range(G)]
+    cb = rnd(G)
+    base = rnd(ln)
+    a, b = base.copy(), base.copy()
+    L.or_gf64_region_coupled_muladd_arr(P(a), ptrs(blocks), P(cb), ln, G)
+    R.gf64_region_coupled_muladd_scalar_arr(P(b), ptrs(blocks), P(cb), ctypes.c_size_t(ln), ctypes.c_size_t(G))
+    assert np.array_equal(a, b)
+    # fused output: one input, K outputs, coefficient per output
+    cs = [rnd(1) for _ in range(K)]
+    outs_a = [rnd(ln) for _ in range(K)]
+    outs_b = [o.copy() for o in outs_a]
+    L.or_gf64_region_fused_output_muladd_arr(ptrs(outs_a), P(x), ptrs(cs), ln, K)
+    R.gf64_region_fused_output_muladd_scalar_arr(ptrs(outs_b), P(x), ptrs(cs), ctypes.c_size_t(ln),
+                                                 ctypes.c_size_t(K))
+    assert all(np.array_equal(p, q) for p, q in zip(outs_a, outs_b))
+    # 2D: K outputs x G input blocks, coefficient matrix with a row stride
+    stride = G + 3
+    c2 = rnd(K * stride)
+    outs_a = [rnd(ln) for _ in range(K)]
+    outs_b = [o.copy() for o in outs_a]
+    L.or_gf64_region_2d_muladd_arr(ptrs(outs_a), K, ptrs(blocks), G, P(c2), stride, ln)
+    R.gf64_region_2d_muladd_scalar_arr(ptrs(outs_b), ctypes.c_size_t(K), ptrs(blocks), ctypes.c_size_t(G),
+                                       P(c2), ctypes.c_size_t(stride), ctypes.c_size_t(ln))
+    assert all(np.array_equal(p, q) for p, q in zip(outs_a, outs_b))
+
+
+def test_par2_columns_equal_create(oracle):
+    """The column evaluator the GPU tests use at full size equals the full PAR2
+    create on every column of a small case (odd slice size included)."""
+    rng = np.random.default_rng(0x7C01)
+    for n, s, r in ((5, 34, 3), (9, 1001, 4)):
+        ins = rng.integers(0, 256, size=(n, s + (s & 1)), dtype=np.uint8)
+        if s & 1:
+            ins[:, s] = 0  # the odd tail byte's partner is zero
+        inums = rng.choice(32768, n, replace=False).astype(np.uint16)
+        exps = rng.choice(65535, r, replace=False).astype(np.uint16)
+        full = oracle.par2_create([ins[i, :s] for i in range(n)], inums.tolist(), exps.tolist(), s)
+        words = (s + 1) // 2
+        cols = oracle.par2_columns(np.ascontiguousarray(ins), ins.shape[1], inums, exps,
+                                   np.arange(words, dtype=np.uint64))
+        # (an odd slice's last word: its low byte is the slice's last byte)
+        assert np.array_equal(np.ascontiguousarray(cols).view(np.uint8)[:, :s], full)
