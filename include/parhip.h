@@ -195,6 +195,20 @@ int parhip_par3_create_streaming(const char *sourcePath, int32_t numRecovery, in
                                  int useMmap, uint8_t **recoveryOut, size_t *recoveryBytes,
                                  double *durationMs, double *throughputMBps);
 
+/* GF64Controller::ComputeRecoveryBlocksFromFile (src/par3_engine.h:104-110,
+ * src/par3_engine.cc:1445-1537): numInputs = file size / (blockSize64 * 8),
+ * the size a positive multiple of it; the numRecovery recovery blocks go into
+ * the caller's buffer.  As in the reference, the call is gated by the
+ * environment variable PAR3_GF64_USE_MMAP=1 and returns 0 on success, -1 on
+ * any failure (gate off, null argument, open/fstat failure, size mismatch);
+ * parhip_last_error() says which.  The file is read in chunks into pinned
+ * memory and reduced on the device while the next chunk is read.  numThreads
+ * is accepted for signature parity. */
+int parhip_gf64_compute_recovery_blocks_from_file(const char *sourcePath, uint64_t *recovery,
+                                                  size_t numRecovery, size_t blockSize64,
+                                                  uint64_t firstInput, uint64_t firstRecovery,
+                                                  int numThreads);
+
 /* ======================================================================= */
 /* PAR3 recovery packets                                                    */
 /* ======================================================================= */

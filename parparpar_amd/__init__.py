@@ -39,6 +39,7 @@ _SIGS = [
      [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp]),
     ("parhip_d_assemble_recovery_packets", _int,
      [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp, _vp]),
+    ("parhip_gf64_compute_recovery_blocks_from_file", _int, [ctypes.c_char_p, _vp, _sz, _sz, _u64, _u64, _int]),
     ("parhip_par3_create_streaming", _int,
      [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, _u64, _u64, ctypes.c_int32, _int,
       ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)), ctypes.POINTER(_sz),

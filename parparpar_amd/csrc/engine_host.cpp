@@ -545,6 +545,34 @@ int parhip_gf64_region_2d_muladd_arr(uint64_t* const* outs, size_t K,
 // ComputeRecoveryBlocksFull.  Here the file is read chunk by chunk straight
 // into pinned staging slots (pread on the host copy threads) and each chunk
 // is reduced on the device while the next one is read.
+// Chunks of a file (blocks of S bytes, `total` bytes in all) read straight
+// into the pinned staging slots by the host copy threads; bytes past the end
+// of the file read as zeros.
+static FillFn file_fill(int fd, size_t total, size_t S, std::atomic<int>* read_err) {
+	return [fd, total, S, read_err](void* dst, uint64_t first, uint64_t count) {
+		const size_t off = (size_t)first * S, bytes = (size_t)count * S, part = 4u << 20;
+		parallel_parts((bytes + part - 1) / part, [&](size_t p) {
+			const size_t o = p * part, len = std::min(part, bytes - o);
+			size_t got = 0;
+			while (got < len && off + o + got < total) {
+				const size_t want = std::min(len - got, total - (off + o + got));
+				const ssize_t r = ::pread(fd, (char*)dst + o + got, want, (off_t)(off + o + got));
+				if (r < 0 && errno == EINTR) continue;
+				if (r <= 0) {
+					int expect = 0;
+					read_err->compare_exchange_strong(expect, r < 0 ? errno : EIO);
+					return;
+				}
+				got += (size_t)r;
+			}
+			if (got < len) memset((char*)dst + o + got, 0, len - got);  // zero tail
+		});
+		const int err = read_err->load();
+		if (err) return fail(PARHIP_ERR_ARG, "read failed: errno=%d (%s)", err, strerror(err));
+		return (int)PARHIP_OK;
+	};
+}
+
 static const char* errno_code(int err) {
 	switch (err) {
 		case ENOENT: return "ENOENT";
@@ -605,29 +633,7 @@ int parhip_par3_create_streaming(const char* sourcePath, int32_t numRecovery, in
 		DevBuf dout;
 		rc = dout.alloc(out_bytes, s);
 		std::atomic<int> read_err{0};
-		const FillFn fill = [&](void* dst, uint64_t first, uint64_t count) {
-			const size_t off = (size_t)first * S, bytes = (size_t)count * S, part = 4u << 20;
-			parallel_parts((bytes + part - 1) / part, [&](size_t p) {
-				const size_t o = p * part, len = std::min(part, bytes - o);
-				size_t got = 0;
-				while (got < len && off + o + got < total) {
-					const size_t want = std::min(len - got, total - (off + o + got));
-					const ssize_t r = ::pread(fd, (char*)dst + o + got, want, (off_t)(off + o + got));
-					if (r < 0 && errno == EINTR) continue;
-					if (r <= 0) {
-						int expect = 0;
-						read_err.compare_exchange_strong(expect, r < 0 ? errno : EIO);
-						return;
-					}
-					got += (size_t)r;
-				}
-				if (got < len) memset((char*)dst + o + got, 0, len - got);  // zero tail
-			});
-			const int err = read_err.load();
-			if (err)
-				return fail(PARHIP_ERR_ARG, "read failed: errno=%d (%s)", err, strerror(err));
-			return (int)PARHIP_OK;
-		};
+		const FillFn fill = file_fill(fd, total, S, &read_err);
 		if (!rc) rc = stream_cauchy(fill, N, W, dout.as<uint64_t>(), R, firstInput, firstRecovery, 0, s);
 		if (!rc) rc = d2h(rec, dout.p, out_bytes, s);
 		if (!rc) rc = sync(s);
@@ -644,6 +650,74 @@ int parhip_par3_create_streaming(const char* sourcePath, int32_t numRecovery, in
 	if (durationMs) *durationMs = ms;
 	if (throughputMBps) *throughputMBps = ms > 0 ? ((double)out_bytes / 1048576.0) / (ms / 1000.0) : 0;
 	return PARHIP_OK;
+}
+
+// GF64Controller::ComputeRecoveryBlocksFromFile (src/par3_engine.cc:1445-1537):
+// the inputs are the file's blocks (its size a positive multiple of the block
+// size), the recovery blocks go into the caller's buffer.  The reference maps
+// the file behind the PAR3_GF64_USE_MMAP=1 gate and returns -1 without it; the
+// same gate and the same -1 for every failure are kept here, while the data
+// path is the chunked read into pinned memory of the streaming create (the
+// device reduces each chunk while the next one is read).
+int parhip_gf64_compute_recovery_blocks_from_file(const char* sourcePath, uint64_t* recovery,
+                                                  size_t numRecovery, size_t blockSize64,
+                                                  uint64_t firstInput, uint64_t firstRecovery,
+                                                  int numThreads) {
+	(void)numThreads;
+	const char* env = std::getenv("PAR3_GF64_USE_MMAP");
+	if (!env || !*env || std::atoi(env) != 1) {
+		fail(PARHIP_ERR_ARG, "PAR3_GF64_USE_MMAP=0; use compute_recovery_blocks with the inputs in memory");
+		return -1;
+	}
+	if (!sourcePath || !recovery) {
+		fail(PARHIP_ERR_ARG, "compute_recovery_blocks_from_file: null sourcePath or recovery");
+		return -1;
+	}
+	if (blockSize64 == 0) {
+		fail(PARHIP_ERR_ARG, "compute_recovery_blocks_from_file: blockSize64 == 0");
+		return -1;
+	}
+	const int fd = ::open(sourcePath, O_RDONLY);
+	if (fd < 0) {
+		const int err = errno;
+		fail(PARHIP_ERR_ARG, "compute_recovery_blocks_from_file: open(%s) failed: %s", sourcePath,
+		     strerror(err));
+		return -1;
+	}
+	struct stat st;
+	if (::fstat(fd, &st) != 0) {
+		const int err = errno;
+		::close(fd);
+		fail(PARHIP_ERR_ARG, "compute_recovery_blocks_from_file: fstat(%s) failed: %s", sourcePath,
+		     strerror(err));
+		return -1;
+	}
+	const size_t total = (size_t)st.st_size, S = blockSize64 * 8;
+	if (blockSize64 > SIZE_MAX / 8 || total < S || total % S != 0) {
+		::close(fd);
+		fail(PARHIP_ERR_ARG,
+		     "compute_recovery_blocks_from_file: file size %zu is not a positive multiple of "
+		     "blockSize64 * 8 = %zu", total, S);
+		return -1;
+	}
+	const uint64_t N = total / S;
+	int rc = PARHIP_OK;
+	if (numRecovery) {
+		size_t out_bytes;
+		hipStream_t s = nullptr;
+		rc = words_bytes(numRecovery, blockSize64, &out_bytes);
+		if (!rc) rc = host_stream(&s);
+		DevBuf dout;
+		if (!rc) rc = dout.alloc(out_bytes, s);
+		std::atomic<int> read_err{0};
+		if (!rc)
+			rc = stream_cauchy(file_fill(fd, total, S, &read_err), N, blockSize64, dout.as<uint64_t>(),
+			                   numRecovery, firstInput, firstRecovery, 0, s);
+		if (!rc) rc = d2h(recovery, dout.p, out_bytes, s);
+		if (!rc) rc = sync(s);
+	}
+	::close(fd);
+	return rc ? -1 : 0;
 }
 
 }  // extern "C"

@@ -126,3 +126,42 @@ def test_create_streaming_short_file_leg5(chunk, oracle, tmp_path, mmap):
                                                     "useMmap": mmap})
     padded = np.frombuffer(data + bytes(4096 - 1000), dtype=np.uint64).reshape(1, 512)
     assert res["recoveryBuffer"] == oracle.recovery(padded, 4, 0, 1).tobytes()
+
+
+def _from_file(path, out, r, w, fi, fr):
+    import ctypes
+    from parparpar_amd import lib
+    p = None if path is None else str(path).encode()
+    return lib().parhip_gf64_compute_recovery_blocks_from_file(
+        p, None if out is None else out.ctypes.data, r, w, fi, fr, 0)
+
+
+@pytest.mark.parametrize("n,bs,r,chunk_bytes", [
+    (37, 8192, 5, 64 << 10),     # several chunks
+    (300, 1 << 20, 8, 0),        # 300 MiB: five default chunks
+    (1, 64, 1, 0),
+])
+def test_from_file_into_caller_buffer(chunk, oracle, tmp_path, monkeypatch, n, bs, r, chunk_bytes):
+    """ComputeRecoveryBlocksFromFile: the file's blocks are the inputs, the
+    recovery blocks land in the caller's buffer; equal to the oracle (small)
+    or to the device-resident create plus oracle columns (300 MiB)."""
+    import torch
+    from parparpar_amd import device
+    chunk(chunk_bytes)
+    monkeypatch.setenv("PAR3_GF64_USE_MMAP", "1")
+    w = bs // 8
+    rng = np.random.default_rng(n * 3 + r)
+    inp = np.frombuffer(rng.bytes(n * bs), dtype=np.uint64).reshape(n, w)
+    path = tmp_path / "src.bin"
+    inp.tofile(path)
+    fi, fr = 3, n + 11
+    out = np.zeros((r, w), dtype=np.uint64)
+    assert _from_file(path, out, r, w, fi, fr) == 0
+    if n * bs <= (4 << 20):
+        assert np.array_equal(out, oracle.recovery_with_coeff(inp, oracle.cauchy(n, r, fi, fr)))
+    else:
+        d = device.recovery(torch.from_numpy(inp.view(np.int64).copy()).cuda(), r, fi, fr)
+        assert np.array_equal(out, d.cpu().numpy().view(np.uint64))
+        cols = np.array([0, 5, w // 2, w - 1])
+        ref = oracle.recovery_with_coeff(np.ascontiguousarray(inp[:, cols]), oracle.cauchy(n, r, fi, fr))
+        assert np.array_equal(out[:, cols], ref)
