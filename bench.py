@@ -13,8 +13,9 @@ recovery slices.
 Prints one JSON line.  `value` is the headline rate; the same line carries the
 rest of the metric, each measured in this run: the 262,144 x 4 KiB geometry,
 the PAR2 GF(2^16) create (1 GiB, 512 recovery slices) with its CPU baseline,
-the batched-vs-plain uneven create (200 jobs) and the repair serving loop's
-p50/p99 against a stated bound.  `roofline` is for the dominant kernel (the
+the batched-vs-plain uneven create (200 jobs), the repair serving loop's
+p50/p99 against a stated bound and the PAR2 backend from host memory
+(PCIe-inclusive).  `roofline` is for the dominant kernel (the
 contraction), timed with HIP events on the stream it runs on; `cpu_baseline`
 times the reference's own CPU engine (oracle/_ref, built from the reference
 sources) on a bounded sample of the same workload on this host's cores.
@@ -393,6 +394,9 @@ def main_par3(args):
         line["batched_uneven_200_jobs"] = measure_batch(args)
         torch.cuda.empty_cache()
         line["repair_serving"] = measure_repair(args)
+        torch.cuda.empty_cache()
+        # the PAR2 backend from host memory, at its default configuration
+        line["par2_host_32768x32KiB_r512"] = measure_par2_host(args, runs=(("default", 0, 2),))
     print(json.dumps(line), flush=True)
 
 
@@ -488,19 +492,19 @@ def main_par2(args):
     print(json.dumps(line), flush=True)
 
 
-def main_par2_host(args):
+def measure_par2_host(args, runs=(("default", 0, 2), ("batch64", 64, 2), ("areas4", 0, 4))):
     """PAR2 create through the backend engine a PAR2 front end drives
-    (parhip_par2_*): 32,768 host slices x 32 KiB, 512 recovery slices; each add
-    claims a staging slot and the copy workers stage the slice (copy +
-    checksum word) into pinned memory, batches go to HBM and through the
-    GF(2^16) kernels, getOutput copies each recovery slice back and verifies
-    its checksum.  PCIe-inclusive; the outputs are checked against the
-    device-resident contraction of the same bytes."""
+    (parhip_par2_*): 1 GiB of host slices (32,768 x 32 KiB by default), 512
+    recovery slices; each add claims a staging slot and the copy workers stage
+    the slice (copy + checksum word) into pinned memory, batches go to HBM and
+    through the GF(2^16) kernels, getOutput copies each recovery slice back and
+    verifies its checksum.  PCIe-inclusive; the outputs are checked against the
+    device-resident contraction of the same bytes.  runs: (name, inputs per
+    batch (0: the engine's choice), staging areas)."""
     import numpy as np
     import torch
     import parparpar_amd as P
     from parparpar_amd.par2 import GfProc
-    torch.cuda.set_device(0)
     L = P.lib()
     r = 512
     sb = args.par2_slice_bytes
@@ -518,12 +522,9 @@ def main_par2_host(args):
     P.check(L.parhip_d_gf16_muladd(x.data_ptr(), n, words, ref.data_ptr(), r, words, words,
                                    co.data_ptr(), 0, sp))
     ref_host = ref.cpu().numpy().view(np.uint8)
-    del x, co
+    del x, co, ref
     res = {}
     out = np.empty((r, sb), dtype=np.uint8)
-    # (inputs per batch, staging areas): the engine's choice (0: from the slice
-    # size) with the reference backends' two staging areas; 64 per batch; four areas
-    runs = (("default", 0, 2), ("batch64", 64, 2), ("areas4", 0, 4))
     for name, batch, areas in runs:
         gp = GfProc(sb, input_batch=batch, staging_areas=areas)
         gp.set_recovery_slices(exps)
@@ -543,20 +544,31 @@ def main_par2_host(args):
         gp.end()
         valid = gp.get_all(out)
         el = time.perf_counter() - t0
-        res[name] = (n * sb / 2 ** 30 / el, valid and np.array_equal(out, ref_host), nb, areas)
+        res[name] = {"GiBps": round(n * sb / 2 ** 30 / el, 3), "inputs_per_batch": nb,
+                     "staging_areas": areas,
+                     "bit_exact_and_checksums_valid": bool(valid and np.array_equal(out, ref_host))}
         gp.close()
+    res["workload"] = ("PAR2 create 1 GiB = %d host slices x %d KiB, 512 recovery slices, asynchronous "
+                       "adds (copy workers), end, get; PCIe-inclusive" % (n, sb >> 10))
+    return res
+
+
+def main_par2_host(args):
+    import torch
+    torch.cuda.set_device(0)
+    res = measure_par2_host(args)
     d = res["default"]
     line = {
         "metric": "GiB/s encoded from host memory through the PAR2 backend (PCIe-inclusive), 1GiB R=512",
-        "value": round(d[0], 3), "unit": "GiB/s", "n_gpus": 1, "steps": 1,
+        "value": d["GiBps"], "unit": "GiB/s", "n_gpus": 1, "steps": 1,
         "higher_is_better": True, "dtype": "u16",
         "data": "synthetic (seeded uniform random bytes in host memory)",
-        "config": {"workload": "PAR2 create 1 GiB = %d host slices x %d KiB, 512 recovery slices, "
-                               "asynchronous adds (copy workers), end, get; %d inputs per batch "
-                               "(the engine's choice), %d staging areas" % (n, sb >> 10, d[2], d[3])},
-        "batch64_GiBps": round(res["batch64"][0], 3),
-        "areas4_GiBps": round(res["areas4"][0], 3),
-        "bit_exact_and_checksums_valid": all(v[1] for v in res.values()),
+        "config": {"workload": res["workload"] + "; %d inputs per batch (the engine's choice), %d staging "
+                                                 "areas" % (d["inputs_per_batch"], d["staging_areas"])},
+        "batch64_GiBps": res["batch64"]["GiBps"],
+        "areas4_GiBps": res["areas4"]["GiBps"],
+        "bit_exact_and_checksums_valid": all(v["bit_exact_and_checksums_valid"] for k, v in res.items()
+                                             if k != "workload"),
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -643,7 +655,8 @@ def measure_batch(args):
         "batched_vs_plain": round(res["plain"] / res["batched"], 3),
         "plan_s": round(plan_s, 3),
         "batch_info": b.info(),
-        "jobs_by_method": {"table": sum(b.job_method(k) == P.METHOD_TABLE for k in range(len(jobs))),
+        "jobs_by_method": {"xor_block": sum(b.job_method(k) == P.METHOD_DYADIC for k in range(len(jobs))),
+                           "table": sum(b.job_method(k) == P.METHOD_TABLE for k in range(len(jobs))),
                            "mfma": sum(b.job_method(k) == P.METHOD_MFMA for k in range(len(jobs))),
                            "mfma_candidates": sum(64 * y.shape[0] <= x.shape[1] for (x, y, _, _) in jobs)},
         "bit_exact_batched_vs_plain": bool(exact),
