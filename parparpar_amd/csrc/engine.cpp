@@ -579,7 +579,11 @@ DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t
 	}
 	const uint64_t tiles = (W + 256ull * p.wpt - 1) / (256ull * p.wpt);
 	const uint64_t base = tiles * p.Q;
-	const uint64_t target = (uint64_t)device_cus(dev) * 2;
+	// one round of resident work-groups: 2 per CU at 4 words per thread (205
+	// VGPRs), 4 at 2 words per thread (127 VGPRs, 28 KiB of LDS each); the
+	// narrow tiles gain from the extra waves (262144 x 4 KiB, R = 8: 0.56 ->
+	// 0.52 ms with the reduce)
+	const uint64_t target = (uint64_t)device_cus(dev) * (p.wpt == 2 ? 4 : 2);
 	uint64_t groups = base < target ? (target + base - 1) / base : 1;
 	if (t_tune_groups > 0) groups = (uint64_t)t_tune_groups;
 	// at least 2 batches of blocks per row, so the table builds stay amortised
