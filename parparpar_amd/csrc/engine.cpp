@@ -579,12 +579,34 @@ DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t
 	}
 	const uint64_t tiles = (W + 256ull * p.wpt - 1) / (256ull * p.wpt);
 	const uint64_t base = tiles * p.Q;
-	// one round of resident work-groups: 2 per CU at 4 words per thread (205
-	// VGPRs), 4 at 2 words per thread (127 VGPRs, 28 KiB of LDS each); the
-	// narrow tiles gain from the extra waves (262144 x 4 KiB, R = 8: 0.56 ->
-	// 0.52 ms with the reduce)
-	const uint64_t target = (uint64_t)device_cus(dev) * (p.wpt == 2 ? 4 : 2);
-	uint64_t groups = base < target ? (target + base - 1) / base : 1;
+	// Resident work-groups: 2 per CU at 4 words per thread (205 VGPRs), 4 at
+	// 2 words per thread (127 VGPRs, 28 KiB of LDS each; 262144 x 4 KiB,
+	// R = 8: 0.56 -> 0.50 ms against 2 per CU).  The split-K row count g
+	// minimises the blocks one slot works through, rounds(g) * blocks/g
+	// (whole rounds: a round that is half full costs a full one), plus the
+	// reduce, which reads g partial rows: 1024 x 1 MiB at R = 24 goes from
+	// 2 rows (768 work-groups, 1.5 rounds: 1.95 ms) to 4 (3 rounds: 1.54 ms).
+	const uint64_t slots = (uint64_t)device_cus(dev) * (p.wpt == 2 ? 4 : 2);
+	const uint64_t gmax = std::min<uint64_t>(std::max<uint64_t>(1, p.nblocks / (2 * p.ib)), 65535);
+	// a slot's block step is ~16 us of 1024-word tile work; the reduce reads
+	// R x W words per row at ~4 TB/s
+	const double row_cost = (double)R * W * 8 / 4e12 / 16e-6;
+	uint64_t groups = 1;
+	double best = 0;
+	auto consider = [&](uint64_t g) {
+		if (g < 1 || g > gmax) return;
+		const uint64_t rounds = (base * g + slots - 1) / slots;
+		const uint64_t per = (p.nblocks + g - 1) / g;
+		const double cost = (double)rounds * per + (g > 1 ? row_cost * g : 0.0);
+		if (cost < best || (cost == best && g < groups)) {
+			best = cost;
+			groups = g;
+		}
+	};
+	best = (double)((base + slots - 1) / slots) * p.nblocks;
+	// small counts, and the counts that fill whole rounds
+	for (uint64_t g = 2; g <= 64; g++) consider(g);
+	for (uint64_t k = 1; k <= 16; k++) consider(k * slots / base);
 	if (t_tune_groups > 0) groups = (uint64_t)t_tune_groups;
 	// at least 2 batches of blocks per row, so the table builds stay amortised
 	groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, p.nblocks / (2 * p.ib)));
