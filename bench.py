@@ -262,6 +262,12 @@ def measure_par3(args, d_in, n, sb, r, steps, warmup):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
+    names = ("mac_dyadic", "mac_tab", "mac_tab8", "mac_mfma", "mfma_rows", "xor_reduce",
+             "dyadic_coeffs", "cauchy")
+    # the timed steps: HIP events around the contraction kernel only (two per
+    # step; events around every kernel add ~20 us to a 0.54 ms step)
+    mac = MAC_NAME[args.method]
+    P.prof_only(mac)
     P.prof_enable(True)
     P.prof_reset()
     torch.cuda.synchronize()
@@ -272,10 +278,20 @@ def measure_par3(args, d_in, n, sb, r, steps, warmup):
     elapsed = time.perf_counter() - t0
     P.prof_enable(False)
     kern = {}
-    for k in ("mac_dyadic", "mac_tab", "mac_tab8", "mac_mfma", "mfma_rows", "xor_reduce",
-              "dyadic_coeffs", "cauchy"):
+    ms, cnt = P.prof_get(mac)
+    if cnt:
+        kern[mac] = ms / cnt
+    # the other kernels of a step, from a short bracket of their own
+    P.prof_only(None)
+    P.prof_enable(True)
+    P.prof_reset()
+    for _ in range(min(steps, 20)):
+        step()
+    torch.cuda.synchronize()
+    P.prof_enable(False)
+    for k in names:
         ms, cnt = P.prof_get(k)
-        if cnt:
+        if cnt and k not in kern:
             kern[k] = ms / cnt
     return n * sb * steps / 2 ** 30 / elapsed, elapsed / steps * 1e3, kern, d_out
 

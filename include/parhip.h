@@ -309,6 +309,9 @@ int parhip_d_gf64_recovery_ptrs(const uint64_t *const *d_in_ptrs, size_t numIn,
 /* ======================================================================= */
 /* Bracket every kernel launched by this thread with HIP events (on its stream). */
 int parhip_prof_enable(int on);
+/* Bracket only the kernels of this name ("mac_dyadic", ...; NULL or "" = all):
+ * two events per step instead of two per kernel. */
+int parhip_prof_only(const char *kernel);
 /* Drop accumulated timings. */
 int parhip_prof_reset(void);
 /* Total device time and launch count of a kernel: "mac_tab", "xor_reduce",

@@ -125,6 +125,7 @@ _SIGS = [
     ("parhip_repair_server_stats", _int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("parhip_repair_server_cache_stats", _int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("parhip_repair_server_destroy", None, [_vp]),
+    ("parhip_prof_only", _int, [ctypes.c_char_p]),
     ("parhip_prof_enable", _int, [_int]),
     ("parhip_prof_reset", _int, []),
     ("parhip_prof_get", _int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
@@ -178,6 +179,11 @@ def set_method(method):
 
 def prof_enable(on=True):
     check(lib().parhip_prof_enable(1 if on else 0))
+
+
+def prof_only(kernel=None):
+    """Time only the kernels named `kernel` (None: all)."""
+    check(lib().parhip_prof_only(kernel.encode() if kernel else None))
 
 
 def prof_reset():

@@ -275,11 +275,12 @@ struct ProfRec {
 	hipEvent_t a, b;
 };
 static thread_local bool t_prof = false;
+static thread_local std::string t_prof_only;  // non-empty: time only this kernel
 static thread_local std::vector<ProfRec> t_prof_pending;
 static thread_local std::map<std::string, std::pair<double, uint64_t>> t_prof_tot;
 
 ProfScope::ProfScope(const char* name, hipStream_t s) : s_(s) {
-	if (!t_prof) return;
+	if (!t_prof || (!t_prof_only.empty() && t_prof_only != name)) return;
 	name_ = name;
 	hipEventCreate(&a_);
 	hipEventCreate(&b_);
@@ -738,6 +739,11 @@ int parhip_set_method(int method) {
 
 int parhip_prof_enable(int on) {
 	t_prof = on != 0;
+	return PARHIP_OK;
+}
+
+int parhip_prof_only(const char* kernel) {
+	t_prof_only = kernel ? kernel : "";
 	return PARHIP_OK;
 }
 
