@@ -108,15 +108,22 @@ __device__ __forceinline__ void dy_nib16x8(uint32_t x, uint32_t (&ad)[8]) {
 #undef DY_LO
 }
 
+// c * x^32: the 32 bits shifted out fold back times x^4 + x^3 + x + 1 (at
+// most 36 bits, no second fold).
+__device__ __forceinline__ uint64_t gf64_mulx32(uint64_t c) {
+	const uint64_t h = c >> 32;
+	return (c << 32) ^ h ^ (h << 1) ^ (h << 3) ^ (h << 4);
+}
+
 // The two coefficients of the table entries this thread builds for block b
-// (thread (tbl, n) with tbl < 7), 0 past the group's blocks.
+// (thread (tbl, half, n) with tbl < 7), 0 past the group's blocks.
 __device__ __forceinline__ void dy_load_coeffs(const DyadicArgs& a, uint64_t q, uint64_t b,
                                                uint64_t b_end, uint64_t& c0, uint64_t& c1) {
 	const int tid = threadIdx.x;
 	c0 = 0;
 	c1 = 0;
-	if (tid < kDyTables * 16 && b < b_end) {
-		const int tbl = tid >> 4;
+	if (tid < kDyTables * 32 && b < b_end) {
+		const int tbl = tid >> 5;
 		const uint64_t* gp = a.gp + (q * a.gp_ld + b) * 8;
 		c0 = dy_load(gp + kDySA[tbl]);
 		c1 = dy_load(gp + kDySB[tbl]);
@@ -272,17 +279,24 @@ __device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_
 	dy_load_pair<WPT, 0, FAST>(a, b_begin, b_end, w0, lost, d);
 	if (WPT == 4) dy_load_pair<WPT, (WPT == 4 ? 1 : 0), FAST>(a, b_begin, b_end, w0, lost, d);
 	for (uint64_t bb = b_begin; bb < b_end; bb++) {
-		// ---- tables: thread (tbl, n) owns entry n of all 16 positions of one
-		// pair table; entry(k, n) = x^4 * entry(k-1, n).  The 16 lanes of a
-		// table write 16 consecutive entries of one row per store.
+		// ---- tables: thread (tbl, half, n) owns entry n of 8 of the 16
+		// positions of one pair table; entry(k, n) = x^4 * entry(k-1, n), the
+		// second half starting from x^32 * entry(0, n).  The 16 lanes of a
+		// (table, half) write 16 consecutive entries of one row per store.
 		__syncthreads();
-		if (tid < kDyTables * 16) {
+		if (tid < kDyTables * 32) {
+			// thread (tbl, half, n): entry n of positions 8*half .. 8*half+7
 			const int n = tid & 15;
-			const int tbl = tid >> 4;
+			const int half = (tid >> 4) & 1;
+			const int tbl = tid >> 5;
 			uint64_t e0 = gf64_mul_nibble(c0, (unsigned)n);
 			uint64_t e1 = gf64_mul_nibble(c1, (unsigned)n);
+			if (half) {
+				e0 = gf64_mulx32(e0);
+				e1 = gf64_mulx32(e1);
+			}
 #pragma unroll
-			for (int k = 0; k < 16; k++) {
+			for (int k = 0; k < 8; k++) {
 				if (k) {
 					e0 = gf64_mulx4(e0);
 					e1 = gf64_mulx4(e1);
@@ -292,7 +306,7 @@ __device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_
 				v.y = (uint32_t)(e0 >> 32);
 				v.z = (uint32_t)e1;
 				v.w = (uint32_t)(e1 >> 32);
-				tab[(tbl * 16 + k) * 16 + n] = v;
+				tab[(tbl * 16 + 8 * half + k) * 16 + n] = v;
 			}
 		}
 		__syncthreads();
