@@ -641,9 +641,12 @@ bool dyadic_applies(const MacArgs& a) {
 }
 
 hipError_t enqueue_dyadic(const MacArgs& a, const DyadicPlan& p, const uint64_t* gp,
-                          uint64_t gp_ld, void* ws, hipStream_t s, const uint32_t* lost) {
+                          uint64_t gp_ld, void* ws, hipStream_t s, const uint32_t* lost,
+                          uint64_t xbase, uint64_t ybase) {
 	DyadicArgs d{};
 	d.lost = lost;
+	d.xbase = xbase;
+	d.ybase = ybase;
 	d.in_base = a.in_base;
 	d.in_stride = a.in_stride;
 	d.out_base = a.out_base;
@@ -686,19 +689,25 @@ int run_cauchy_contraction(MacArgs a, uint64_t fi, uint64_t fr, hipStream_t s, u
 		const uint64_t cfi = fi + first;
 		const uint64_t b0 = ((cfi & ~7ull) - (fi & ~7ull)) / 8;
 		const DyadicPlan p = plan_dyadic(a.N, a.R, a.W, cfi, fr, t_device);
+		// cached coefficients where the cache is on; else the kernel computes
+		// them itself (one inversion per thread per 32 blocks: no separate
+		// launch, no coefficient memory)
 		const uint64_t* gp = nullptr;
-		int rc = coeff_acquire_kind(1, n_total, a.R, fi, fr, s, &gp);
-		if (rc) return rc;
+		if (t_coeff_cache) {
+			int rc = coeff_acquire_kind(1, n_total, a.R, fi, fr, s, &gp);
+			if (rc) return rc;
+		}
 		void* ws = nullptr;
-		rc = dev_alloc(&ws, p.part_bytes, s);
+		int rc = dev_alloc(&ws, p.part_bytes, s);
 		if (rc) {
-			coeff_release(gp, s);
+			if (gp) coeff_release(gp, s);
 			return rc;
 		}
 		const uint64_t call_blocks = ((fi & 7) + n_total + 7) / 8;
-		const hipError_t e = enqueue_dyadic(a, p, gp + b0 * 8, call_blocks, ws, s);
+		const hipError_t e = enqueue_dyadic(a, p, gp ? gp + b0 * 8 : nullptr, call_blocks, ws, s, nullptr,
+		                                    cfi & ~7ull, fr & ~7ull);
 		dev_free(ws, s);
-		coeff_release(gp, s);
+		if (gp) coeff_release(gp, s);
 		return e == hipSuccess ? PARHIP_OK : hip_fail(e, "k_mac_dyadic");
 	}
 	const uint64_t* coeff = nullptr;

@@ -70,8 +70,11 @@ bool dyadic_shape_ok(uint64_t R);
 // gp: the coefficients of the launch's first block; gp_ld: blocks per output
 // group row of the coefficient array; lost: optional per-block masks of inputs
 // that count as zero (DyadicArgs::lost).
+// gp null: the kernel computes the coefficients from the points xbase + 8b,
+// ybase + 8q (both multiples of 8).
 hipError_t enqueue_dyadic(const MacArgs& a, const DyadicPlan& p, const uint64_t* gp,
-                          uint64_t gp_ld, void* ws, hipStream_t s, const uint32_t* lost = nullptr);
+                          uint64_t gp_ld, void* ws, hipStream_t s, const uint32_t* lost = nullptr,
+                          uint64_t xbase = 0, uint64_t ybase = 0);
 // The plain Cauchy contraction of a call with n_total inputs at points
 // (fi + i, fr + r), run over its inputs [first, first + a.N): the dyadic kernel
 // where it applies, else the cached coefficient matrix and run_contraction.

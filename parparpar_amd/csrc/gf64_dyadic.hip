@@ -13,15 +13,16 @@ namespace parhip {
 // blockIdx.z.  FAST: every tile is full and 16-byte aligned (checked by the
 // launcher).  MASK: inputs flagged in a.lost count as zero.
 template <int WPT, bool FAST, bool MASK>
-__global__ __launch_bounds__(256, 2) void k_mac_dyadic(DyadicArgs a) {
+__global__ __launch_bounds__(256, WPT == 2 ? 4 : 2) void k_mac_dyadic(DyadicArgs a) {
 	// tab[(tbl*16 + k)*16 + n] = (g'_SA * n x^{4k}, g'_SB * n x^{4k})
 	__shared__ uint4 tab[kDyTables * 256];
+	__shared__ uint64_t coef[kDyChunk * 8];  // coefficients computed here (a.gp null)
 	const uint64_t g = blockIdx.y;
 	const uint64_t b_begin = g * a.blocks_per_group;
 	uint64_t b_end = b_begin + a.blocks_per_group;
 	if (b_end > a.nblocks) b_end = a.nblocks;
-	dy_tile<WPT, FAST, MASK>(a, tab, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end, g,
-	                         a.partial ? DY_OUT_PARTIAL : DY_OUT_DIRECT);
+	dy_tile<WPT, FAST, MASK>(a, tab, coef, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end,
+	                         g, a.partial ? DY_OUT_PARTIAL : DY_OUT_DIRECT);
 }
 
 // Transformed coefficients of every (output group q, input block b):

@@ -115,18 +115,54 @@ __device__ __forceinline__ uint64_t gf64_mulx32(uint64_t c) {
 	return (c << 32) ^ h ^ (h << 1) ^ (h << 3) ^ (h << 4);
 }
 
+// Coefficients computed in the kernel (a.gp null) come in chunks of
+// kDyChunk blocks, kept in LDS.
+constexpr int kDyChunk = 32;
+
+// The transformed coefficients of blocks [b0, b0 + kDyChunk) of output group q
+// into coef (thread 8p + u: block b0 + p, slot u; one inversion each, the
+// superset sums across the 8 lanes of a block).  Every thread takes part.
+__device__ __forceinline__ void dy_chunk_coeffs(const DyadicArgs& a, uint64_t* coef, uint64_t q,
+                                                uint64_t b0, uint64_t b_end) {
+	const int tid = threadIdx.x;
+	if (tid < kDyChunk * 8) {
+		const int u = tid & 7;
+		const uint64_t b = b0 + (uint64_t)(tid >> 3);
+		uint64_t c = 0;
+		if (b < b_end) {
+			const uint64_t z = ((a.xbase + 8 * b) ^ (a.ybase + 8 * q)) ^ (uint64_t)u;
+			c = z ? gf64_inv(z) : 1u;  // the reference's 0 -> 1 (src/par3_engine.cc:392-394)
+		}
+#pragma unroll
+		for (int bit = 1; bit < 8; bit <<= 1) {
+			const uint32_t lo = __shfl_xor((uint32_t)c, bit, 64);
+			const uint32_t hi = __shfl_xor((uint32_t)(c >> 32), bit, 64);
+			if (!(u & bit)) c ^= ((uint64_t)hi << 32) | lo;
+		}
+		coef[tid] = c;
+	}
+}
+
 // The two coefficients of the table entries this thread builds for block b
-// (thread (tbl, half, n) with tbl < 7), 0 past the group's blocks.
-__device__ __forceinline__ void dy_load_coeffs(const DyadicArgs& a, uint64_t q, uint64_t b,
-                                               uint64_t b_end, uint64_t& c0, uint64_t& c1) {
+// (thread (tbl, half, n) with tbl < 7), 0 past the group's blocks: from the
+// coefficient array, or from the chunk in LDS (b_begin: the tile's first block).
+__device__ __forceinline__ void dy_load_coeffs(const DyadicArgs& a, const uint64_t* coef, uint64_t q,
+                                               uint64_t b, uint64_t b_begin, uint64_t b_end,
+                                               uint64_t& c0, uint64_t& c1) {
 	const int tid = threadIdx.x;
 	c0 = 0;
 	c1 = 0;
 	if (tid < kDyTables * 32 && b < b_end) {
 		const int tbl = tid >> 5;
-		const uint64_t* gp = a.gp + (q * a.gp_ld + b) * 8;
-		c0 = dy_load(gp + kDySA[tbl]);
-		c1 = dy_load(gp + kDySB[tbl]);
+		if (a.gp) {
+			const uint64_t* gp = a.gp + (q * a.gp_ld + b) * 8;
+			c0 = dy_load(gp + kDySA[tbl]);
+			c1 = dy_load(gp + kDySB[tbl]);
+		} else {
+			const uint64_t* cp = coef + ((b - b_begin) % kDyChunk) * 8;
+			c0 = cp[kDySA[tbl]];
+			c1 = cp[kDySB[tbl]];
+		}
 	}
 }
 
@@ -258,8 +294,9 @@ __device__ __forceinline__ void dy_pair_step(const DyadicArgs& a, const uint4* t
 enum { DY_OUT_DIRECT = 0, DY_OUT_PARTIAL = 1, DY_OUT_ATOMIC = 2 };
 
 template <int WPT, bool FAST, bool MASK = false>
-__device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_t q, uint64_t w0,
-                                        uint64_t b_begin, uint64_t b_end, uint64_t g, int out_mode) {
+__device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_t* coef, uint64_t q,
+                                        uint64_t w0, uint64_t b_begin, uint64_t b_end, uint64_t g,
+                                        int out_mode) {
 	static_assert(WPT == 2 || WPT == 4, "WPT");
 	const int tid = threadIdx.x;
 	uint2 acc[WPT][8];
@@ -274,7 +311,11 @@ __device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_
 	// pairs' lookups and the next table build.
 	uint2 d[8][WPT];
 	uint64_t c0, c1;
-	dy_load_coeffs(a, q, b_begin, b_end, c0, c1);
+	if (!a.gp) {
+		dy_chunk_coeffs(a, coef, q, b_begin, b_end);
+		__syncthreads();
+	}
+	dy_load_coeffs(a, coef, q, b_begin, b_begin, b_end, c0, c1);
 	uint32_t lost = dy_lost<MASK>(a, b_begin, b_end);
 	dy_load_pair<WPT, 0, FAST>(a, b_begin, b_end, w0, lost, d);
 	if (WPT == 4) dy_load_pair<WPT, (WPT == 4 ? 1 : 0), FAST>(a, b_begin, b_end, w0, lost, d);
@@ -310,7 +351,13 @@ __device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_
 			}
 		}
 		__syncthreads();
-		dy_load_coeffs(a, q, bb + 1, b_end, c0, c1);
+		if (!a.gp && bb + 1 < b_end && (bb + 1 - b_begin) % kDyChunk == 0) {
+			// (every read of the previous chunk is done: its last values were
+			// used by the build above)
+			dy_chunk_coeffs(a, coef, q, bb + 1, b_end);
+			__syncthreads();
+		}
+		dy_load_coeffs(a, coef, q, bb + 1, b_begin, b_end, c0, c1);
 		lost = dy_lost<MASK>(a, bb + 1, b_end);
 		dy_pair_step<WPT, 0, FAST>(a, tab, bb, b_end, w0, lost, d, acc);
 		if (WPT == 4) dy_pair_step<WPT, (WPT == 4 ? 1 : 0), FAST>(a, tab, bb, b_end, w0, lost, d, acc);

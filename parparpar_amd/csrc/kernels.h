@@ -90,6 +90,9 @@ struct DyadicArgs {
 	// if non-null: bit t of lost[b] set = the input at virtual slot 8b+t
 	// counts as zero whatever its memory holds (the known inputs of a repair)
 	const uint32_t* lost;
+	// gp null: the kernel computes the coefficients itself from the points,
+	// X_b = xbase + 8b and Y_q = ybase + 8q (multiples of 8)
+	uint64_t xbase, ybase;
 };
 hipError_t launch_dyadic_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uint64_t xbase,
                                 uint64_t ybase, hipStream_t s);
