@@ -562,6 +562,9 @@ int run_contraction_tables(MacArgs a, hipStream_t s) {
 // for one round of work-groups per resident slot (2 per CU at 205 / 129
 // VGPRs): at 1024 x 1 MiB, R = 8, 4 rows measured 0.530 + 0.010 ms (kernel +
 // reduce) against 0.535 + 0.015 ms for 8.
+// Split-K rows from which the XOR-block launches fold with atomics.
+static const uint64_t kDyAtomicRows = 16;
+
 DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev,
                        size_t max_part_bytes) {
 	DyadicPlan p;
@@ -613,13 +616,16 @@ DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t
 	groups = std::min<uint64_t>(groups, std::max<uint64_t>(1, p.nblocks / (2 * p.ib)));
 	groups = std::min<uint64_t>(groups, 65535);
 	// fewer rows where the caller's partial buffer is smaller (one row needs none)
-	while (groups > 1 && (size_t)xor_reduce_scratch_words(groups, R, W) * sizeof(uint64_t) > max_part_bytes)
+	while (groups > 1 && groups < kDyAtomicRows &&
+	       (size_t)xor_reduce_scratch_words(groups, R, W) * sizeof(uint64_t) > max_part_bytes)
 		groups--;
 	uint64_t per = (p.nblocks + groups - 1) / groups;
 	per = (per + p.ib - 1) / p.ib * p.ib;
 	p.groups = (p.nblocks + per - 1) / per;
 	p.blocks_per_group = per;
-	p.part_bytes = p.groups > 1 ? (size_t)xor_reduce_scratch_words(p.groups, R, W) * sizeof(uint64_t) : 0;
+	p.part_bytes = p.groups > 1 && p.groups < kDyAtomicRows
+	                   ? (size_t)xor_reduce_scratch_words(p.groups, R, W) * sizeof(uint64_t)
+	                   : 0;
 	return p;
 }
 
@@ -660,14 +666,24 @@ hipError_t enqueue_dyadic(const MacArgs& a, const DyadicPlan& p, const uint64_t*
 	d.pre_out = p.pre_out;
 	d.nblocks = p.nblocks;
 	d.blocks_per_group = p.blocks_per_group;
-	d.partial = p.groups > 1 ? (uint64_t*)ws : nullptr;
+	// many split-K rows (narrow slices: 262144 x 4 KiB has 1024): the rows XOR
+	// into the zeroed outputs with atomics (0.514 -> 0.504 ms with the zeroing)
+	// instead of partials and a reduce that reads all of them; a few rows (C2a:
+	// 4): partials and the reduce
+	const bool atomic = p.groups >= kDyAtomicRows;
+	d.partial = p.groups > 1 && !atomic ? (uint64_t*)ws : nullptr;
 	d.accumulate = a.accumulate;
-	hipError_t e;
-	{
+	d.atomic_out = atomic;
+	hipError_t e = hipSuccess;
+	if (atomic && !a.accumulate) {
+		ProfScope ps("zero_rows", s);
+		e = launch_zero_rows(a.out_base, a.R, a.W, a.out_stride, s);
+	}
+	if (e == hipSuccess) {
 		ProfScope ps("mac_dyadic", s);
 		e = launch_mac_dyadic(d, p.groups, p.wpt, p.ib, s);
 	}
-	if (e == hipSuccess && p.groups > 1) {
+	if (e == hipSuccess && p.groups > 1 && !atomic) {
 		ProfScope ps("xor_reduce", s);
 		e = launch_xor_reduce(d.partial, p.groups, a.R, a.W, a.out_base, a.out_stride, nullptr,
 		                      a.accumulate, s);

@@ -22,7 +22,7 @@ __global__ __launch_bounds__(256, WPT == 2 ? 4 : 2) void k_mac_dyadic(DyadicArgs
 	uint64_t b_end = b_begin + a.blocks_per_group;
 	if (b_end > a.nblocks) b_end = a.nblocks;
 	dy_tile<WPT, FAST, MASK>(a, tab, coef, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end,
-	                         g, a.partial ? DY_OUT_PARTIAL : DY_OUT_DIRECT);
+	                         g, a.partial ? DY_OUT_PARTIAL : (a.atomic_out ? DY_OUT_ATOMIC : DY_OUT_DIRECT));
 }
 
 // Transformed coefficients of every (output group q, input block b):
@@ -63,6 +63,34 @@ hipError_t launch_dyadic_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uin
 	if (blocks > 65536) blocks = 65536;
 	hipLaunchKernelGGL(k_dyadic_coeffs, dim3((unsigned)blocks), dim3(256), 0, s, out, nblocks, Q,
 	                   xbase, ybase);
+	return hipGetLastError();
+}
+
+// Rows of `words` words at `stride` set to zero (a kernel: safe to record in
+// a graph, where memset nodes are not).
+__global__ void k_zero_rows(uint64_t* __restrict__ out, uint64_t rows, uint64_t words, uint64_t stride) {
+	const uint64_t r = blockIdx.y;
+	for (uint64_t w = 2 * ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x); w < words;
+	     w += 2 * (uint64_t)gridDim.x * blockDim.x) {
+		uint64_t* p = out + r * stride + w;
+		if (w + 1 < words && ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)) {
+			*reinterpret_cast<ulonglong2*>(p) = make_ulonglong2(0, 0);
+		} else {
+			p[0] = 0;
+			if (w + 1 < words) p[1] = 0;
+		}
+	}
+}
+
+hipError_t launch_zero_rows(uint64_t* out, uint64_t rows, uint64_t words, uint64_t stride,
+                            hipStream_t s) {
+	if (!rows || !words) return hipSuccess;
+	if (rows > 65535) return hipErrorInvalidValue;
+	uint64_t bx = (words / 2 + 255) / 256;
+	if (bx > 1024) bx = 1024;
+	if (!bx) bx = 1;
+	hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)bx, (unsigned)rows), dim3(256), 0, s, out, rows, words,
+	                   stride);
 	return hipGetLastError();
 }
 

@@ -93,7 +93,12 @@ struct DyadicArgs {
 	// gp null: the kernel computes the coefficients itself from the points,
 	// X_b = xbase + 8b and Y_q = ybase + 8q (multiples of 8)
 	uint64_t xbase, ybase;
+	// split-K rows XOR their results into the outputs with atomics (zeroed
+	// first unless accumulating) instead of writing partials
+	int atomic_out;
 };
+hipError_t launch_zero_rows(uint64_t* out, uint64_t rows, uint64_t words, uint64_t stride,
+                            hipStream_t s);
 hipError_t launch_dyadic_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uint64_t xbase,
                                 uint64_t ybase, hipStream_t s);
 bool mac_dyadic_supported(int wpt, int ib);
