@@ -11,8 +11,9 @@ namespace parhip {
 
 // Work-group: word tile blockIdx.x, split-K row blockIdx.y, output group
 // blockIdx.z.  FAST: every tile is full and 16-byte aligned (checked by the
-// launcher).  MASK: inputs flagged in a.lost count as zero.
-template <int WPT, bool FAST, bool MASK>
+// launcher).  MASK: inputs flagged in a.lost count as zero.  GP: coefficients
+// from a.gp (else computed here).
+template <int WPT, bool FAST, bool MASK, bool GP, bool PIPE = false>
 __global__ __launch_bounds__(256, WPT == 2 ? 4 : 2) void k_mac_dyadic(DyadicArgs a) {
 	// tab[(tbl*16 + k)*16 + n] = (g'_SA * n x^{4k}, g'_SB * n x^{4k})
 	__shared__ uint4 tab[kDyTables * 256];
@@ -21,7 +22,7 @@ __global__ __launch_bounds__(256, WPT == 2 ? 4 : 2) void k_mac_dyadic(DyadicArgs
 	const uint64_t b_begin = g * a.blocks_per_group;
 	uint64_t b_end = b_begin + a.blocks_per_group;
 	if (b_end > a.nblocks) b_end = a.nblocks;
-	dy_tile<WPT, FAST, MASK>(a, tab, coef, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end,
+	dy_tile<WPT, FAST, MASK, GP, PIPE>(a, tab, coef, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end,
 	                         g, a.partial ? DY_OUT_PARTIAL : (a.atomic_out ? DY_OUT_ATOMIC : DY_OUT_DIRECT));
 }
 
@@ -105,16 +106,36 @@ hipError_t launch_mac_dyadic(const DyadicArgs& a, uint64_t groups, int wpt, int 
 	const bool fast = ((reinterpret_cast<uintptr_t>(a.in_base) & 15u) == 0) && (a.in_stride & 1) == 0 &&
 	                  a.W % (256ull * wpt) == 0;
 	if (ib != 1 || (wpt != 4 && wpt != 2)) return hipErrorInvalidValue;
-	const int v = (wpt == 4 ? 4 : 0) | (fast ? 2 : 0) | (a.lost ? 1 : 0);
+	const int v = (wpt == 4 ? 8 : 0) | (fast ? 4 : 0) | (a.lost ? 2 : 0) | (a.gp ? 1 : 0);
+	if (v == 12 && Q == 1) {
+		// one output group (R <= 8): tables read one step ahead
+		hipLaunchKernelGGL((k_mac_dyadic<4, true, false, false, true>), grid, dim3(256), 0, s, a);
+		return hipGetLastError();
+	}
+	if (v == 13 && Q == 1) {
+		hipLaunchKernelGGL((k_mac_dyadic<4, true, false, true, true>), grid, dim3(256), 0, s, a);
+		return hipGetLastError();
+	}
 	switch (v) {
-		case 6: hipLaunchKernelGGL((k_mac_dyadic<4, true, false>), grid, dim3(256), 0, s, a); break;
-		case 7: hipLaunchKernelGGL((k_mac_dyadic<4, true, true>), grid, dim3(256), 0, s, a); break;
-		case 4: hipLaunchKernelGGL((k_mac_dyadic<4, false, false>), grid, dim3(256), 0, s, a); break;
-		case 5: hipLaunchKernelGGL((k_mac_dyadic<4, false, true>), grid, dim3(256), 0, s, a); break;
-		case 2: hipLaunchKernelGGL((k_mac_dyadic<2, true, false>), grid, dim3(256), 0, s, a); break;
-		case 3: hipLaunchKernelGGL((k_mac_dyadic<2, true, true>), grid, dim3(256), 0, s, a); break;
-		case 0: hipLaunchKernelGGL((k_mac_dyadic<2, false, false>), grid, dim3(256), 0, s, a); break;
-		default: hipLaunchKernelGGL((k_mac_dyadic<2, false, true>), grid, dim3(256), 0, s, a); break;
+#define DY_CASE(V, W, F, M, G) \
+	case V: hipLaunchKernelGGL((k_mac_dyadic<W, F, M, G>), grid, dim3(256), 0, s, a); break;
+		DY_CASE(15, 4, true, true, true)
+		DY_CASE(14, 4, true, true, false)
+		DY_CASE(13, 4, true, false, true)
+		DY_CASE(12, 4, true, false, false)
+		DY_CASE(11, 4, false, true, true)
+		DY_CASE(10, 4, false, true, false)
+		DY_CASE(9, 4, false, false, true)
+		DY_CASE(8, 4, false, false, false)
+		DY_CASE(7, 2, true, true, true)
+		DY_CASE(6, 2, true, true, false)
+		DY_CASE(5, 2, true, false, true)
+		DY_CASE(4, 2, true, false, false)
+		DY_CASE(3, 2, false, true, true)
+		DY_CASE(2, 2, false, true, false)
+		DY_CASE(1, 2, false, false, true)
+		default: hipLaunchKernelGGL((k_mac_dyadic<2, false, false, false>), grid, dim3(256), 0, s, a); break;
+#undef DY_CASE
 	}
 	return hipGetLastError();
 }

@@ -346,7 +346,7 @@ __device__ __forceinline__ void run_batch_dy(const BatchJob& jb, const BatchTile
 	a.pre_out = jb.pre_out;
 	a.nblocks = jb.nblocks;
 	a.accumulate = 0;
-	dy_tile<WPT, true>(a, tab, nullptr, tl.r0, tl.w0, tl.i0, tl.i1, 0,
+	dy_tile<WPT, true, false, true>(a, tab, nullptr, tl.r0, tl.w0, tl.i0, tl.i1, 0,
 	                   tl.mode == OUT_ATOMIC ? DY_OUT_ATOMIC : DY_OUT_DIRECT);
 }
 
