@@ -144,6 +144,8 @@ METHOD_MFMA = 2
 METHOD_MFMA_I8 = 3
 METHOD_TABLE8 = 4
 METHOD_DYADIC = 5
+METHOD_DYADIC8 = 6
+METHOD_DYADIC16 = 7
 
 
 class ParhipError(RuntimeError):

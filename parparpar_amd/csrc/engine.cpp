@@ -149,9 +149,12 @@ static std::list<CoeffKey> g_coeff_lru;
 static const size_t kCoeffCacheMax = 16;
 
 // Words of the dyadic coefficients for (N, R, fi, fr).
-static uint64_t dyadic_coeff_words(uint64_t n, uint64_t r, uint64_t fi, uint64_t fr) {
-	const uint64_t nblocks = ((fi & 7) + n + 7) / 8, Q = ((fr & 7) + r + 7) / 8;
-	return nblocks * Q * 8;
+// kind 1: XOR-block coefficients of 8 x 8 blocks (8 words per (q, b));
+// kind 2: of 16 x 16 blocks (16 words per (q, b)).
+static uint64_t dyadic_coeff_words(int kind, uint64_t n, uint64_t r, uint64_t fi, uint64_t fr) {
+	const uint64_t m = kind == 2 ? 15 : 7;
+	const uint64_t nblocks = ((fi & m) + n + m) / (m + 1), Q = ((fr & m) + r + m) / (m + 1);
+	return nblocks * Q * (m + 1);
 }
 
 static int coeff_acquire_kind(int kind, uint64_t n, uint64_t r, uint64_t fi, uint64_t fr,
@@ -165,6 +168,11 @@ int coeff_acquire(uint64_t n, uint64_t r, uint64_t fi, uint64_t fr, hipStream_t 
 // Build a coefficient set of `kind` into `d` on stream s.
 static hipError_t coeff_build(int kind, uint64_t* d, uint64_t n, uint64_t r, uint64_t fi,
                               uint64_t fr, hipStream_t s) {
+	if (kind == 2) {
+		ProfScope ps("dyadic_coeffs", s);
+		return launch_dyadic16_coeffs(d, ((fi & 15) + n + 15) / 16, ((fr & 15) + r + 15) / 16,
+		                              fi & ~15ull, fr & ~15ull, s);
+	}
 	if (kind) {
 		ProfScope ps("dyadic_coeffs", s);
 		return launch_dyadic_coeffs(d, ((fi & 7) + n + 7) / 8, ((fr & 7) + r + 7) / 8, fi & ~7ull,
@@ -180,7 +188,7 @@ static thread_local bool t_coeff_cache = true;
 
 static int coeff_acquire_kind(int kind, uint64_t n, uint64_t r, uint64_t fi, uint64_t fr,
                               hipStream_t s, const uint64_t** out) {
-	const uint64_t words = kind ? dyadic_coeff_words(n, r, fi, fr) : n * r;
+	const uint64_t words = kind ? dyadic_coeff_words(kind, n, r, fi, fr) : n * r;
 	if (!t_coeff_cache) {
 		void* p = nullptr;
 		int rc = dev_alloc(&p, words * sizeof(uint64_t), s);
@@ -566,19 +574,33 @@ int run_contraction_tables(MacArgs a, hipStream_t s) {
 static const uint64_t kDyAtomicRows = 16;
 
 DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev,
-                       size_t max_part_bytes) {
+                       size_t max_part_bytes, bool allow16) {
 	DyadicPlan p;
-	p.pre_in = fi & 7;
-	p.pre_out = fr & 7;
-	p.nblocks = (p.pre_in + N + 7) / 8;
-	p.Q = (p.pre_out + R + 7) / 8;
-	p.wpt = W >= 1024 ? 4 : 2;
+	// Block size: 16 x 16 blocks take 81 products per 256 input-output pairs,
+	// 8 x 8 blocks 27 per 64; per input that is Q16 * 81/16 against
+	// Q8 * 27/8 (R = 16, 32, 64: a quarter less; R = 24: equal; R <= 8: 8 x 8).
+	if (allow16 && t_method != PARHIP_METHOD_DYADIC8) {
+		const uint64_t q8 = ((fr & 7) + R + 7) / 8, q16 = ((fr & 15) + R + 15) / 16;
+		if (t_method == PARHIP_METHOD_DYADIC16 || (double)q16 * 81 / 16 < 0.97 * (double)q8 * 27 / 8)
+			p.bits = 4;
+	}
+	const uint64_t m = p.bits == 4 ? 15 : 7;
+	p.pre_in = fi & m;
+	p.pre_out = fr & m;
+	p.nblocks = (p.pre_in + N + m) / (m + 1);
+	p.Q = (p.pre_out + R + m) / (m + 1);
 	p.ib = 1;
-	if (t_tune_wpt || t_tune_ib) {
-		const int wpt = t_tune_wpt ? t_tune_wpt : p.wpt, ib = t_tune_ib ? t_tune_ib : p.ib;
-		if (mac_dyadic_supported(wpt, ib)) {
-			p.wpt = wpt;
-			p.ib = ib;
+	if (p.bits == 4) {
+		p.wpt = 1;
+		if (t_tune_wpt == 1 || t_tune_wpt == 2) p.wpt = t_tune_wpt;
+	} else {
+		p.wpt = W >= 1024 ? 4 : 2;
+		if (t_tune_wpt || t_tune_ib) {
+			const int wpt = t_tune_wpt ? t_tune_wpt : p.wpt, ib = t_tune_ib ? t_tune_ib : p.ib;
+			if (mac_dyadic_supported(wpt, ib)) {
+				p.wpt = wpt;
+				p.ib = ib;
+			}
 		}
 	}
 	const uint64_t tiles = (W + 256ull * p.wpt - 1) / (256ull * p.wpt);
@@ -590,11 +612,14 @@ DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t
 	// (whole rounds: a round that is half full costs a full one), plus the
 	// reduce, which reads g partial rows: 1024 x 1 MiB at R = 24 goes from
 	// 2 rows (768 work-groups, 1.5 rounds: 1.95 ms) to 4 (3 rounds: 1.54 ms).
-	const uint64_t slots = (uint64_t)device_cus(dev) * (p.wpt == 2 ? 4 : 2);
+	// (16 x 16 blocks: 58 KiB of LDS, two work-groups per CU at either wpt)
+	const uint64_t slots = (uint64_t)device_cus(dev) * (p.bits == 3 && p.wpt == 2 ? 4 : 2);
 	const uint64_t gmax = std::min<uint64_t>(std::max<uint64_t>(1, p.nblocks / (2 * p.ib)), 65535);
-	// a slot's block step is ~16 us of 1024-word tile work; the reduce reads
-	// R x W words per row at ~4 TB/s
-	const double row_cost = (double)R * W * 8 / 4e12 / 16e-6;
+	// a slot's block step is ~16 us of 1024-word tile work (8 x 8 blocks; a
+	// 16 x 16 block of a 256-word tile is 3/4 of that per word); the reduce
+	// reads R x W words per row at ~4 TB/s
+	const double step_s = p.bits == 4 ? 16e-6 * 0.75 * p.wpt : 16e-6;
+	const double row_cost = (double)R * W * 8 / 4e12 / step_s;
 	uint64_t groups = 1;
 	double best = 0;
 	auto consider = [&](uint64_t g) {
@@ -638,7 +663,9 @@ bool dyadic_shape_ok(uint64_t R) {
 }
 
 bool dyadic_applies_shape(uint64_t R) {
-	if (t_method != PARHIP_METHOD_AUTO && t_method != PARHIP_METHOD_DYADIC) return false;
+	if (t_method != PARHIP_METHOD_AUTO && t_method != PARHIP_METHOD_DYADIC &&
+	    t_method != PARHIP_METHOD_DYADIC8 && t_method != PARHIP_METHOD_DYADIC16)
+		return false;
 	return dyadic_shape_ok(R);
 }
 
@@ -679,7 +706,10 @@ hipError_t enqueue_dyadic(const MacArgs& a, const DyadicPlan& p, const uint64_t*
 		ProfScope ps("zero_rows", s);
 		e = launch_zero_rows(a.out_base, a.R, a.W, a.out_stride, s);
 	}
-	if (e == hipSuccess) {
+	if (e == hipSuccess && p.bits == 4) {
+		ProfScope ps("mac_dyadic16", s);
+		e = launch_mac_dyadic16(d, p.groups, p.wpt, s);
+	} else if (e == hipSuccess) {
 		ProfScope ps("mac_dyadic", s);
 		e = launch_mac_dyadic(d, p.groups, p.wpt, p.ib, s);
 	}
@@ -703,14 +733,15 @@ int run_cauchy_contraction(MacArgs a, uint64_t fi, uint64_t fr, hipStream_t s, u
 	if (dyadic_applies(a)) {
 		// the chunk's own virtual blocks start at block b0 of the call's
 		const uint64_t cfi = fi + first;
-		const uint64_t b0 = ((cfi & ~7ull) - (fi & ~7ull)) / 8;
-		const DyadicPlan p = plan_dyadic(a.N, a.R, a.W, cfi, fr, t_device);
+		const DyadicPlan p = plan_dyadic(a.N, a.R, a.W, cfi, fr, t_device, SIZE_MAX, true);
+		const uint64_t m = p.bits == 4 ? 15 : 7;
+		const uint64_t b0 = ((cfi & ~m) - (fi & ~m)) / (m + 1);
 		// cached coefficients where the cache is on; else the kernel computes
-		// them itself (one inversion per thread per 32 blocks: no separate
-		// launch, no coefficient memory)
+		// them itself (one inversion per thread per chunk of blocks: no
+		// separate launch, no coefficient memory)
 		const uint64_t* gp = nullptr;
 		if (t_coeff_cache) {
-			int rc = coeff_acquire_kind(1, n_total, a.R, fi, fr, s, &gp);
+			int rc = coeff_acquire_kind(p.bits == 4 ? 2 : 1, n_total, a.R, fi, fr, s, &gp);
 			if (rc) return rc;
 		}
 		void* ws = nullptr;
@@ -719,12 +750,12 @@ int run_cauchy_contraction(MacArgs a, uint64_t fi, uint64_t fr, hipStream_t s, u
 			if (gp) coeff_release(gp, s);
 			return rc;
 		}
-		const uint64_t call_blocks = ((fi & 7) + n_total + 7) / 8;
-		const hipError_t e = enqueue_dyadic(a, p, gp ? gp + b0 * 8 : nullptr, call_blocks, ws, s, nullptr,
-		                                    cfi & ~7ull, fr & ~7ull);
+		const uint64_t call_blocks = ((fi & m) + n_total + m) / (m + 1);
+		const hipError_t e = enqueue_dyadic(a, p, gp ? gp + b0 * (m + 1) : nullptr, call_blocks, ws, s,
+		                                    nullptr, cfi & ~m, fr & ~m);
 		dev_free(ws, s);
 		if (gp) coeff_release(gp, s);
-		return e == hipSuccess ? PARHIP_OK : hip_fail(e, "k_mac_dyadic");
+		return e == hipSuccess ? PARHIP_OK : hip_fail(e, p.bits == 4 ? "k_mac_dyadic16" : "k_mac_dyadic");
 	}
 	const uint64_t* coeff = nullptr;
 	int rc = coeff_acquire(n_total, a.R, fi, fr, s, &coeff);
@@ -756,7 +787,7 @@ int parhip_set_device(int device) {
 }
 
 int parhip_set_method(int method) {
-	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_DYADIC)
+	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_DYADIC16)
 		return fail(PARHIP_ERR_ARG, "parhip_set_method: unknown method %d", method);
 	t_method = method;
 	return PARHIP_OK;
@@ -788,7 +819,9 @@ int parhip_prof_get(const char* kernel, double* total_ms, uint64_t* launches) {
 }
 
 int parhip_tune_override(int rt, int wpt, int ib, int64_t groups) {
-	if ((rt || wpt || ib) && !mac_tab_supported(rt ? rt : 8, wpt ? wpt : 4, ib ? ib : 4) &&
+	// (rt = ib = 0, wpt 1 / 2: the 16 x 16 XOR-block form's tile widths)
+	const bool dy16 = !rt && !ib && (wpt == 1 || wpt == 2);
+	if ((rt || wpt || ib) && !dy16 && !mac_tab_supported(rt ? rt : 8, wpt ? wpt : 4, ib ? ib : 4) &&
 	    !mac_tab8_supported(rt ? rt : 4, wpt ? wpt : 4, ib ? ib : 1) &&
 	    !gf16_mfma_supported(rt ? rt : 4, wpt ? wpt : 4, true) &&
 	    !gf16_mfma_supported(rt ? rt : 4, wpt ? wpt : 2, false) &&

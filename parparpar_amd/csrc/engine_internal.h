@@ -52,6 +52,9 @@ hipError_t enqueue_mfma(const MacArgs& m, const MfmaPlan& p, void* ws, hipStream
 
 // The XOR-block (dyadic) form of the plain Cauchy contraction (gf64_dyadic.hip).
 struct DyadicPlan {
+	// bits 3: 8 x 8 blocks (k_mac_dyadic); 4: 16 x 16 blocks (k_mac_dyadic16).
+	// pre_in, pre_out, nblocks, Q count in runs of 2^bits points.
+	int bits = 3;
 	uint64_t pre_in, pre_out, nblocks, Q;
 	int wpt, ib;
 	uint64_t groups, blocks_per_group;
@@ -59,8 +62,11 @@ struct DyadicPlan {
 };
 // max_part_bytes: the largest split-K partial buffer the caller can give
 // (fewer split-K rows below it).
+// allow16: the caller can run the 16 x 16 form (coefficients computed in the
+// kernel or in the 16-word layout, no lost masks); the plan takes it where it
+// needs fewer table reads.
 DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev,
-                       size_t max_part_bytes = SIZE_MAX);
+                       size_t max_part_bytes = SIZE_MAX, bool allow16 = false);
 // Whether `a` (strided slices, this thread's method) takes the dyadic kernel.
 bool dyadic_applies(const MacArgs& a);
 // Whether R outputs take it under this thread's method (any input layout);

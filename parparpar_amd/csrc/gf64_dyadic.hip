@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gf64_dyadic16_dev.h"
 #include "gf64_dyadic_dev.h"
 
 namespace parhip {
@@ -24,6 +25,85 @@ __global__ __launch_bounds__(256, WPT == 2 ? 4 : 2) void k_mac_dyadic(DyadicArgs
 	if (b_end > a.nblocks) b_end = a.nblocks;
 	dy_tile<WPT, FAST, MASK, GP, PIPE>(a, tab, coef, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end,
 	                         g, a.partial ? DY_OUT_PARTIAL : (a.atomic_out ? DY_OUT_ATOMIC : DY_OUT_DIRECT));
+}
+
+// The 16 x 16 form (gf64_dyadic16_dev.h): same grid, blocks and groups of 16
+// points; 58 KiB of LDS, two work-groups per CU.
+template <int WPT, bool FAST, bool GP>
+__global__ __launch_bounds__(256, 2) void k_mac_dyadic16(DyadicArgs a) {
+	__shared__ uint4 tab[2 * kDyTables * 256];   // two sets of seven pair tables
+	__shared__ uint64_t coef[kD16Chunk * 16];    // coefficients computed here (GP false)
+	const uint64_t g = blockIdx.y;
+	const uint64_t b_begin = g * a.blocks_per_group;
+	uint64_t b_end = b_begin + a.blocks_per_group;
+	if (b_end > a.nblocks) b_end = a.nblocks;
+	d16_tile<WPT, FAST, GP>(a, tab, coef, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end, g,
+	                        a.partial ? DY_OUT_PARTIAL : (a.atomic_out ? DY_OUT_ATOMIC : DY_OUT_DIRECT));
+}
+
+// Transformed coefficients of every (output group q, input block b) of the
+// 16 x 16 form: g(u) = f(X_b XOR Y_q XOR u), X_b = xbase + 16b,
+// Y_q = ybase + 16q, out[(q*nblocks + b)*16 + S] = XOR over supersets T of S
+// of g(T); 16 (q, b) pairs per work-group.
+__global__ __launch_bounds__(256) void k_dyadic16_coeffs(uint64_t* __restrict__ out, uint64_t nblocks,
+                                                         uint64_t Q, uint64_t xbase, uint64_t ybase) {
+	__shared__ uint64_t gs[256];
+	const uint64_t total = nblocks * Q;
+	const int u = threadIdx.x & 15;
+	for (uint64_t base_pair = (uint64_t)blockIdx.x * 16; base_pair < total;
+	     base_pair += (uint64_t)gridDim.x * 16) {
+		const uint64_t idx = base_pair + (threadIdx.x >> 4);
+		uint64_t c = 0;
+		if (idx < total) {
+			const uint64_t q = idx / nblocks, b = idx - q * nblocks;
+			const uint64_t z = ((xbase + 16 * b) ^ (ybase + 16 * q)) ^ (uint64_t)u;
+			c = z ? gf64_inv(z) : 1u;
+		}
+		gs[threadIdx.x] = c;
+		__syncthreads();
+		uint64_t acc = 0;
+#pragma unroll
+		for (int t = 0; t < 16; t++)
+			if ((t & u) == u) acc ^= gs[(threadIdx.x & ~15) + t];
+		if (idx < total) out[idx * 16 + u] = acc;
+		__syncthreads();
+	}
+}
+
+hipError_t launch_dyadic16_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uint64_t xbase,
+                                  uint64_t ybase, hipStream_t s) {
+	const uint64_t total = nblocks * Q;
+	if (!total) return hipSuccess;
+	uint64_t blocks = (total + 15) / 16;
+	if (blocks > 65536) blocks = 65536;
+	hipLaunchKernelGGL(k_dyadic16_coeffs, dim3((unsigned)blocks), dim3(256), 0, s, out, nblocks, Q,
+	                   xbase, ybase);
+	return hipGetLastError();
+}
+
+hipError_t launch_mac_dyadic16(const DyadicArgs& a, uint64_t groups, int wpt, hipStream_t s) {
+	const uint64_t Q = (a.pre_out + a.R + 15) / 16;
+	const uint64_t tiles = (a.W + 256ull * wpt - 1) / (256ull * wpt);
+	if (!tiles || !groups || !Q) return hipSuccess;
+	if (tiles > 0x7fffffffu || groups > 65535 || Q > 65535) return hipErrorInvalidValue;
+	if (a.lost || (wpt != 1 && wpt != 2)) return hipErrorInvalidValue;
+	dim3 grid((unsigned)tiles, (unsigned)groups, (unsigned)Q);
+	const bool fast = a.W % (256ull * wpt) == 0;
+	const int v = (wpt == 2 ? 4 : 0) | (fast ? 2 : 0) | (a.gp ? 1 : 0);
+	switch (v) {
+#define D16_CASE(V, W, F, G) \
+	case V: hipLaunchKernelGGL((k_mac_dyadic16<W, F, G>), grid, dim3(256), 0, s, a); break;
+		D16_CASE(7, 2, true, true)
+		D16_CASE(6, 2, true, false)
+		D16_CASE(5, 2, false, true)
+		D16_CASE(4, 2, false, false)
+		D16_CASE(3, 1, true, true)
+		D16_CASE(2, 1, true, false)
+		D16_CASE(1, 1, false, true)
+		default: hipLaunchKernelGGL((k_mac_dyadic16<1, false, false>), grid, dim3(256), 0, s, a); break;
+#undef D16_CASE
+	}
+	return hipGetLastError();
 }
 
 // Transformed coefficients of every (output group q, input block b):

@@ -103,6 +103,11 @@ hipError_t launch_dyadic_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uin
                                 uint64_t ybase, hipStream_t s);
 bool mac_dyadic_supported(int wpt, int ib);
 hipError_t launch_mac_dyadic(const DyadicArgs& a, uint64_t groups, int wpt, int ib, hipStream_t s);
+// The 16 x 16 form: blocks and output groups of 16 points, gp with 16 words
+// per (q, b); wpt 1 or 2; no lost masks.
+hipError_t launch_dyadic16_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uint64_t xbase,
+                                  uint64_t ybase, hipStream_t s);
+hipError_t launch_mac_dyadic16(const DyadicArgs& a, uint64_t groups, int wpt, hipStream_t s);
 hipError_t launch_cauchy_points(uint64_t* out, const uint64_t* xs, uint64_t nx,
                                 const uint64_t* ys, uint64_t ny, hipStream_t s);
 hipError_t launch_mac_tab(const MacArgs& a, uint64_t groups, int rt, int wpt, int ib,

@@ -58,13 +58,20 @@ def test_dyadic_equals_oracle(gpu, oracle, n, w, r, fi, fr, acc):
     d_out = torch.from_numpy(init.view(np.int64).copy()).cuda()
     expect = init.copy() if acc else np.zeros((r, w), dtype=np.uint64)
     oracle.recovery_with_coeff(inp, oracle.cauchy(n, r, fi, fr), expect, acc)
-    for m in (gpu.METHOD_DYADIC, gpu.METHOD_TABLE, gpu.METHOD_AUTO):
-        _method(gpu, m)
-        out = d_out.clone()
-        device.recovery(d_in, r, fi, fr, out=out, accumulate=acc)
-        torch.cuda.synchronize()
-        assert np.array_equal(out.cpu().numpy().view(np.uint64), expect), (m, n, w, r, fi, fr)
-    _method(gpu, gpu.METHOD_AUTO)
+    from parparpar_amd import lib
+    try:
+        for cache in (1, 0):  # coefficients from the cache / computed in the kernel
+            gpu.check(lib().parhip_set_coeff_cache(cache))
+            for m in (gpu.METHOD_DYADIC, gpu.METHOD_DYADIC8, gpu.METHOD_DYADIC16, gpu.METHOD_TABLE,
+                      gpu.METHOD_AUTO):
+                _method(gpu, m)
+                out = d_out.clone()
+                device.recovery(d_in, r, fi, fr, out=out, accumulate=acc)
+                torch.cuda.synchronize()
+                assert np.array_equal(out.cpu().numpy().view(np.uint64), expect), (cache, m, n, w, r, fi, fr)
+    finally:
+        gpu.check(lib().parhip_set_coeff_cache(1))
+        _method(gpu, gpu.METHOD_AUTO)
 
 
 @pytest.mark.parametrize("wpt,ib", [(4, 1), (2, 1), (2, 2)])
@@ -87,6 +94,33 @@ def test_dyadic_launch_shapes(gpu, oracle, wpt, ib, groups):
         gpu.check(lib().parhip_tune_override(0, 0, 0, 0))
         _method(gpu, gpu.METHOD_AUTO)
     assert np.array_equal(out.cpu().numpy().view(np.uint64), expect)
+
+
+@pytest.mark.parametrize("wpt", [1, 2])
+@pytest.mark.parametrize("groups", [1, 3, 40])
+@pytest.mark.parametrize("cache", [1, 0])
+def test_dyadic16_launch_shapes(gpu, oracle, wpt, groups, cache):
+    """The 16 x 16 form: both tile widths, split-K rows through partials
+    (3) and atomics (40), coefficients from the cache or the kernel; ragged
+    and full tiles (2560 = 5 x 512 words, 1000 words)."""
+    import torch
+    from parparpar_amd import device, lib
+    for n, w, r, fi, fr in ((331, 2560, 20, 5, 331), (97, 1000, 33, 13, 2 ** 40 + 7)):
+        rng = np.random.default_rng(wpt * 100 + groups + n)
+        inp = rand_words(rng, n, w)
+        expect = oracle.recovery(inp, r, fi, fr)
+        d_in = torch.from_numpy(inp.view(np.int64)).cuda()
+        _method(gpu, gpu.METHOD_DYADIC16)
+        try:
+            gpu.check(lib().parhip_set_coeff_cache(cache))
+            gpu.check(lib().parhip_tune_override(0, wpt, 0, groups))
+            out = device.recovery(d_in, r, fi, fr)
+            torch.cuda.synchronize()
+        finally:
+            gpu.check(lib().parhip_tune_override(0, 0, 0, 0))
+            gpu.check(lib().parhip_set_coeff_cache(1))
+            _method(gpu, gpu.METHOD_AUTO)
+        assert np.array_equal(out.cpu().numpy().view(np.uint64), expect), (n, w, r, fi, fr)
 
 
 def test_dyadic_reference_collision_shape(gpu, oracle):
