@@ -41,9 +41,12 @@ struct CfgInfo {
 // per stage of the fp4 kernel's default shape
 static const CfgInfo kCfg[BATCH_CFG_COUNT] = {
 	{8, 4, 4}, {8, 2, 4}, {4, 2, 4} /* MFMA */, {16, 2, 2}, {4, 4, 4},
-	{8, 4, 1} /* dyadic, one 8-output group per tile */, {8, 2, 1} /* dyadic */};
+	{8, 4, 1} /* dyadic, one 8-output group per tile */, {8, 2, 1} /* dyadic */,
+	{16, 1, 1} /* dyadic 16 x 16, one 16-output group per tile */};
 
-static bool is_dyadic(int cfg) { return cfg == BATCH_CFG_DY4 || cfg == BATCH_CFG_DY2; }
+static bool is_dyadic(int cfg) {
+	return cfg == BATCH_CFG_DY4 || cfg == BATCH_CFG_DY2 || cfg == BATCH_CFG_DY16;
+}
 
 // XOR-block tiles run the straight-line (full, 16-byte aligned tile) form of
 // the kernel: slices a multiple of 512 words, inputs 16-byte aligned at an
@@ -54,7 +57,31 @@ static bool dyadic_ok(const BatchJob& jb) {
 	       (jb.in_stride & 1) == 0;
 }
 static int dyadic_default(const BatchJob& jb) {
+	if (jb.bits == 4) return BATCH_CFG_DY16;
 	return jb.W % 1024 == 0 ? BATCH_CFG_DY4 : BATCH_CFG_DY2;
+}
+
+// A dyadic job's block size, fixed when the batch is made (its coefficient
+// layout depends on it): 16 x 16 blocks where they need fewer table reads
+// (the plain create's rule) and the slices fill 256-word tiles.
+static void set_job_geometry(BatchJob& jb) {
+	jb.bits = (dyadic_ok(jb) && jb.W % 256 == 0 && dyadic16_better(jb.R, jb.fr)) ? 4 : 3;
+	const uint64_t m = jb.bits == 4 ? 15 : 7;
+	jb.pre_in = jb.fi & m;
+	jb.pre_out = jb.fr & m;
+	jb.nblocks = (jb.pre_in + jb.N + m) / (m + 1);
+	jb.Q = (jb.pre_out + jb.R + m) / (m + 1);
+}
+
+// The job's coefficient units (8 words each) from unit `off`: its first unit
+// (even for 16 x 16 blocks) and the next free unit.
+static uint64_t gp_units(const BatchJob& jb, uint64_t off, uint64_t* first) {
+	if (jb.bits == 4) {
+		*first = (off + 1) & ~1ull;
+		return *first + 2 * jb.Q * jb.nblocks;
+	}
+	*first = off;
+	return off + jb.Q * jb.nblocks;
 }
 
 // The MFMA strategy needs the 64 bit-matrix rows of every coefficient (512 B
@@ -96,11 +123,14 @@ static const uint64_t kTileWork = 1ull << 22;  // word x output MACs per tile
 
 // XOR-block tiles: a block is 8 inputs x 8 outputs per word; a tile is
 // about twice a table tile's MACs (the method takes ~half the time per MAC).
+// A 16 x 16 block has 3x the products of an 8 x 8 one: as many products per
+// tile.
 static uint32_t plan_job_dyadic(uint32_t jid, const BatchJob& jb, int cfg,
                                 std::vector<BatchTile>& out, bool* atomic) {
 	const uint64_t tw = 256ull * kCfg[cfg].wpt;
 	const uint64_t wt = jb.W / tw;
 	uint64_t chunk = std::max<uint64_t>(2, (2 * kTileWork) / (tw * 64));
+	if (cfg == BATCH_CFG_DY16) chunk = std::max<uint64_t>(2, (2 * kTileWork) / (tw * 64) * 27 / 81);
 	if (chunk >= jb.nblocks) chunk = jb.nblocks;
 	const uint64_t nch = (jb.nblocks + chunk - 1) / chunk;
 	*atomic = nch > 1;
@@ -191,10 +221,11 @@ static int run_single(const BatchJob& jb, int cfg, BatchJob* djob, BatchTile* dt
 	hipEventCreate(&a);
 	hipEventCreate(&b);
 	float best = 1e30f;
+	uint32_t fam[kBatchFamilies] = {0, 0, 0};
+	fam[batch_family(cfg)] = (uint32_t)tiles.size();
 	for (int rep = 0; rep < 3; rep++) {
 		hipEventRecord(a, s);
-		hipError_t e = launch_batch(djob, 1, dtiles, (uint32_t)tiles.size(), dcounter, dzero,
-		                            atomic ? 1 : 0, 0, 0, blocks, s);
+		hipError_t e = launch_batch(djob, 1, dtiles, fam, dcounter, dzero, atomic ? 1 : 0, 0, 0, blocks, s);
 		hipEventRecord(b, s);
 		if (e != hipSuccess) {
 			(void)hipStreamSynchronize(s);  // the tile copy above reads `tiles`
@@ -218,7 +249,7 @@ static int run_single(const BatchJob& jb, int cfg, BatchJob* djob, BatchTile* dt
 // (dyadic jobs).
 static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_out) {
 	const bool dy = dyadic_ok(jb);
-	TuneKey key{jb.W, jb.R, n_bucket(jb.N), dy ? 1 : 0};
+	TuneKey key{jb.W, jb.R, n_bucket(jb.N), dy ? (jb.bits == 4 ? 2 : 1) : 0};
 	{
 		std::lock_guard<std::mutex> lk(g_tune_mu);
 		auto it = g_tune.find(key);
@@ -231,9 +262,14 @@ static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_
 	if (dy) {
 		// the XOR-block method does ~half the LDS work of the direct tables
 		// per MAC at every R measured (4-64); only its tile width is tuned
+		// (16 x 16 blocks: one width)
 		cand.clear();
-		if (jb.W % 1024 == 0) cand.push_back(BATCH_CFG_DY4);
-		cand.push_back(BATCH_CFG_DY2);
+		if (jb.bits == 4) {
+			cand.push_back(BATCH_CFG_DY16);
+		} else {
+			if (jb.W % 1024 == 0) cand.push_back(BATCH_CFG_DY4);
+			cand.push_back(BATCH_CFG_DY2);
+		}
 	}
 	const bool try_mfma = mfma_eligible(jb);
 	if (cand.size() == 1 && !try_mfma) {
@@ -247,7 +283,7 @@ static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_
 	BatchJob j = jb;
 	int rc;
 	if ((rc = dout.alloc(jb.R * jb.W * sizeof(uint64_t), s)) ||
-	    (rc = djob.alloc(sizeof(BatchJob), s)) || (rc = dcnt.alloc(sizeof(uint32_t), s)) ||
+	    (rc = djob.alloc(sizeof(BatchJob), s)) || (rc = dcnt.alloc(kBatchFamilies * sizeof(uint32_t), s)) ||
 	    (rc = dzero.alloc(sizeof(uint32_t), s)))
 		return rc;
 	j.out = dout.as<uint64_t>();
@@ -285,7 +321,7 @@ static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_
 		DyadicPlan dp{};
 		DevBuf dws;
 		if (dy) {
-			dp = plan_dyadic(jb.N, jb.R, jb.W, jb.fi, jb.fr, current_device());
+			dp = plan_dyadic(jb.N, jb.R, jb.W, jb.fi, jb.fr, current_device(), SIZE_MAX, jb.bits == 4);
 			if ((rc = dws.alloc(dp.part_bytes, s))) return rc;
 		}
 		float ms_tab = 1e30f, ms_mfma = 1e30f;
@@ -340,7 +376,8 @@ struct parhip_batch {
 	BatchJob* djobs = nullptr;
 	BatchTile* dtiles = nullptr;
 	uint32_t ntiles = 0;
-	uint32_t* dcounter = nullptr;
+	uint32_t fam_tiles[kBatchFamilies] = {0, 0, 0};  // tiles of each family, grouped in dtiles
+	uint32_t* dcounter = nullptr;                      // one queue counter per family
 	uint32_t* dzero = nullptr;
 	uint32_t nzero = 0;
 	uint64_t* dcoeff = nullptr;
@@ -363,7 +400,7 @@ struct parhip_batch {
 // The batch's launch sequence: every job's coefficients, zeroing of split
 // jobs, queue reset and persistent kernel, then the MFMA jobs' launches.
 static hipError_t batch_enqueue(const parhip_batch* b, hipStream_t s) {
-	hipError_t e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), b->dtiles, b->ntiles, b->dcounter,
+	hipError_t e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), b->dtiles, b->fam_tiles, b->dcounter,
 	                            b->dzero, b->nzero, b->coeff_total, b->dy_pairs, b->blocks, s);
 	for (size_t k = 0; k < b->mfma.size() && e == hipSuccess; k++) {
 		const auto& mj = b->mfma[k];
@@ -431,10 +468,7 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 		jb.W = in.blockSize64;
 		jb.fi = in.firstInput;
 		jb.fr = in.firstRecovery;
-		jb.pre_in = jb.fi & 7;
-		jb.pre_out = jb.fr & 7;
-		jb.nblocks = (jb.pre_in + jb.N + 7) / 8;
-		jb.Q = (jb.pre_out + jb.R + 7) / 8;
+		set_job_geometry(jb);
 		// storage for every form the job may run in: XOR-block coefficients
 		// where it qualifies, the full matrix where tables or MFMA may run it
 		const bool dy = dyadic_ok(jb);
@@ -442,7 +476,7 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 		jb.coeff_off = b->coeff_total;
 		if (full) b->coeff_total += jb.N * jb.R;
 		jb.gp_off = b->dy_pairs;
-		if (dy) b->dy_pairs += jb.Q * jb.nblocks;
+		if (dy) b->dy_pairs = gp_units(jb, b->dy_pairs, &jb.gp_off);
 		b->slot[j] = (int64_t)b->jobs.size();
 		b->jobs.push_back(jb);
 	}
@@ -451,7 +485,7 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 	if (b->coeff_total) e = hipMalloc((void**)&b->dcoeff, b->coeff_total * sizeof(uint64_t));
 	if (e == hipSuccess && b->dy_pairs) e = hipMalloc((void**)&b->dgp, b->dy_pairs * 8 * sizeof(uint64_t));
 	if (e == hipSuccess) e = hipMalloc((void**)&b->djobs, b->jobs.size() * sizeof(BatchJob));
-	if (e == hipSuccess) e = hipMalloc((void**)&b->dcounter, sizeof(uint32_t));
+	if (e == hipSuccess) e = hipMalloc((void**)&b->dcounter, kBatchFamilies * sizeof(uint32_t));
 	if (e != hipSuccess) {
 		hip_fail(e, "batch allocation");
 		batch_free(b);
@@ -464,7 +498,7 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 	e = hipMemcpy(b->djobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice);
 	// coefficients first: the autotuner runs the real kernels on them
 	if (e == hipSuccess)
-		e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), nullptr, 0, b->dcounter, nullptr, 0,
+		e = launch_batch(b->djobs, (uint32_t)b->jobs.size(), nullptr, nullptr, b->dcounter, nullptr, 0,
 		                 b->coeff_total, b->dy_pairs, 0, b->cap);
 	if (e == hipSuccess) e = hipStreamSynchronize(b->cap);
 	if (e != hipSuccess) {
@@ -511,7 +545,7 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 		jb.coeff_off = b->coeff_total;
 		jb.gp_off = b->dy_pairs;
 		if (is_dyadic(b->cfg[j])) {
-			b->dy_pairs += jb.Q * jb.nblocks;
+			b->dy_pairs = gp_units(jb, b->dy_pairs, &jb.gp_off);
 		} else {
 			jb.coeff = b->dcoeff + jb.coeff_off;
 			b->coeff_total += jb.N * jb.R;
@@ -543,10 +577,19 @@ parhip_batch* parhip_batch_create(const parhip_gf64_job* jobs, size_t njobs, uns
 		return nullptr;
 	}
 	tiles.reserve(total);
-	std::vector<size_t> pos(per.size(), 0);
-	while (tiles.size() < total)
+	// grouped by family (each its own persistent kernel), jobs interleaved
+	// inside a family
+	for (int f = 0; f < kBatchFamilies; f++) {
+		size_t fam_total = 0;
 		for (size_t j = 0; j < per.size(); j++)
-			if (pos[j] < per[j].size()) tiles.push_back(per[j][pos[j]++]);
+			if (b->cfg[j] != BATCH_CFG_MFMA && batch_family(b->cfg[j]) == f) fam_total += per[j].size();
+		std::vector<size_t> pos(per.size(), 0);
+		const size_t start = tiles.size();
+		while (tiles.size() - start < fam_total)
+			for (size_t j = 0; j < per.size(); j++)
+				if (batch_family(b->cfg[j]) == f && pos[j] < per[j].size()) tiles.push_back(per[j][pos[j]++]);
+		b->fam_tiles[f] = (uint32_t)fam_total;
+	}
 	b->ntiles = (uint32_t)tiles.size();
 	b->nzero = (uint32_t)zero_ids.size();
 	if (!tiles.empty()) e = hipMalloc((void**)&b->dtiles, tiles.size() * sizeof(BatchTile));
@@ -630,7 +673,8 @@ void parhip_batch_destroy(parhip_batch* b) { batch_free(b); }
 // The entry of a shape: the XOR-block one where the shape qualifies for it
 // (aligned slices of a multiple of 512 words, R >= 4), else the other.
 static std::map<TuneKey, TuneVal>::iterator tune_find(size_t N, size_t R, size_t W) {
-	auto it = g_tune.find(TuneKey{W, R, n_bucket(N), 1});
+	auto it = g_tune.find(TuneKey{W, R, n_bucket(N), 2});
+	if (it == g_tune.end()) it = g_tune.find(TuneKey{W, R, n_bucket(N), 1});
 	if (it == g_tune.end()) it = g_tune.find(TuneKey{W, R, n_bucket(N), 0});
 	return it;
 }
@@ -684,7 +728,8 @@ int parhip_autotune_load(const char* path) {
 		double ns;
 		// an XOR-block variant only under a key that qualifies for it
 		if (sscanf(line, "%llu %llu %llu %d %d %lg", &W, &R, &nb, &dy, &cfg, &ns) == 6 && cfg >= 0 &&
-		    cfg < BATCH_CFG_COUNT && kCfg[cfg].rt && (dy == 0 || dy == 1) && (dy || !is_dyadic(cfg)))
+		    cfg < BATCH_CFG_COUNT && kCfg[cfg].rt && dy >= 0 && dy <= 2 &&
+		    (is_dyadic(cfg) ? (cfg == BATCH_CFG_DY16) == (dy == 2) && dy : true))
 			g_tune[TuneKey{W, R, nb, dy}] = {cfg, ns};
 	}
 	fclose(f);

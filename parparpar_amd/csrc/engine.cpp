@@ -573,17 +573,20 @@ int run_contraction_tables(MacArgs a, hipStream_t s) {
 // Split-K rows from which the XOR-block launches fold with atomics.
 static const uint64_t kDyAtomicRows = 16;
 
+bool dyadic16_better(uint64_t R, uint64_t fr) {
+	if (t_method == PARHIP_METHOD_DYADIC8) return false;
+	if (t_method == PARHIP_METHOD_DYADIC16) return true;
+	const uint64_t q8 = ((fr & 7) + R + 7) / 8, q16 = ((fr & 15) + R + 15) / 16;
+	return (double)q16 * 81 / 16 < 0.97 * (double)q8 * 27 / 8;
+}
+
 DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev,
                        size_t max_part_bytes, bool allow16) {
 	DyadicPlan p;
 	// Block size: 16 x 16 blocks take 81 products per 256 input-output pairs,
 	// 8 x 8 blocks 27 per 64; per input that is Q16 * 81/16 against
 	// Q8 * 27/8 (R = 16, 32, 64: a quarter less; R = 24: equal; R <= 8: 8 x 8).
-	if (allow16 && t_method != PARHIP_METHOD_DYADIC8) {
-		const uint64_t q8 = ((fr & 7) + R + 7) / 8, q16 = ((fr & 15) + R + 15) / 16;
-		if (t_method == PARHIP_METHOD_DYADIC16 || (double)q16 * 81 / 16 < 0.97 * (double)q8 * 27 / 8)
-			p.bits = 4;
-	}
+	if (allow16 && dyadic16_better(R, fr)) p.bits = 4;
 	const uint64_t m = p.bits == 4 ? 15 : 7;
 	p.pre_in = fi & m;
 	p.pre_out = fr & m;

@@ -62,6 +62,9 @@ struct DyadicPlan {
 };
 // max_part_bytes: the largest split-K partial buffer the caller can give
 // (fewer split-K rows below it).
+// Whether 16 x 16 XOR blocks need fewer table reads than 8 x 8 ones for R
+// outputs from point fr (or this thread's method forces either).
+bool dyadic16_better(uint64_t R, uint64_t fr);
 // allow16: the caller can run the 16 x 16 form (coefficients computed in the
 // kernel or in the 16-word layout, no lost masks); the plan takes it where it
 // needs fewer table reads.

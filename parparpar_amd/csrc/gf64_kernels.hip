@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "gf64_dev.h"
+#include "gf64_dyadic16_dev.h"
 #include "gf64_dyadic_dev.h"
 #include "kernels.h"
 
@@ -350,10 +351,34 @@ __device__ __forceinline__ void run_batch_dy(const BatchJob& jb, const BatchTile
 	                   tl.mode == OUT_ATOMIC ? DY_OUT_ATOMIC : DY_OUT_DIRECT);
 }
 
+// A 16 x 16 XOR-block tile (gf64_dyadic16_dev.h): 256 words, output group r0
+// of 16 points, blocks [i0, i1) of 16 points; its 56 KiB of tables fit `tab`.
+__device__ __forceinline__ void run_batch_dy16(const BatchJob& jb, const BatchTile& tl, uint4* tab) {
+	DyadicArgs a{};
+	a.in_base = jb.in;
+	a.in_stride = jb.in_stride;
+	a.out_base = jb.out;
+	a.out_stride = jb.out_stride;
+	a.gp = jb.gp;
+	a.gp_ld = jb.nblocks;
+	a.N = jb.N;
+	a.R = jb.R;
+	a.W = jb.W;
+	a.pre_in = jb.pre_in;
+	a.pre_out = jb.pre_out;
+	a.nblocks = jb.nblocks;
+	a.accumulate = 0;
+	d16_tile<1, true, true>(a, tab, nullptr, tl.r0, tl.w0, tl.i0, tl.i1, 0,
+	                        tl.mode == OUT_ATOMIC ? DY_OUT_ATOMIC : DY_OUT_DIRECT);
+}
+
+// One family of tiles (batch_family): FAM 0 the table variants, 1 the 8 x 8
+// XOR-block tiles, 2 the 16 x 16 ones.
+template <int FAM>
 __global__ __launch_bounds__(256, 2) void k_batch_persistent(const BatchJob* __restrict__ jobs,
                                                           const BatchTile* __restrict__ tiles,
                                                           uint32_t ntiles, uint32_t* counter) {
-	__shared__ uint4 tab[4096];  // 64 KiB: the largest IB*(RT/2)*256 of the variants
+	__shared__ uint4 tab[4096];  // 64 KiB: the largest table set of the family's variants
 	__shared__ uint32_t t_next;
 	for (;;) {
 		if (threadIdx.x == 0) t_next = atomicAdd(counter, 1u);
@@ -363,20 +388,27 @@ __global__ __launch_bounds__(256, 2) void k_batch_persistent(const BatchJob* __r
 		if (t >= ntiles) break;
 		const BatchTile tl = tiles[t];
 		const BatchJob jb = jobs[tl.job];
-		switch (tl.cfg) {
-			case BATCH_CFG_8_4_4: run_batch_tile<8, 4, 4>(jb, tl, tab); break;
-			case BATCH_CFG_8_2_4: run_batch_tile<8, 2, 4>(jb, tl, tab); break;
-			case BATCH_CFG_16_2_2: run_batch_tile<16, 2, 2>(jb, tl, tab); break;
-			case BATCH_CFG_4_4_4: run_batch_tile<4, 4, 4>(jb, tl, tab); break;
-			case BATCH_CFG_DY4: run_batch_dy<4>(jb, tl, tab); break;
-			case BATCH_CFG_DY2: run_batch_dy<2>(jb, tl, tab); break;
-			default: break;
+		if constexpr (FAM == 0) {
+			switch (tl.cfg) {
+				case BATCH_CFG_8_4_4: run_batch_tile<8, 4, 4>(jb, tl, tab); break;
+				case BATCH_CFG_8_2_4: run_batch_tile<8, 2, 4>(jb, tl, tab); break;
+				case BATCH_CFG_16_2_2: run_batch_tile<16, 2, 2>(jb, tl, tab); break;
+				case BATCH_CFG_4_4_4: run_batch_tile<4, 4, 4>(jb, tl, tab); break;
+				default: break;
+			}
+		} else if constexpr (FAM == 1) {
+			if (tl.cfg == BATCH_CFG_DY4)
+				run_batch_dy<4>(jb, tl, tab);
+			else
+				run_batch_dy<2>(jb, tl, tab);
+		} else {
+			run_batch_dy16(jb, tl, tab);
 		}
 	}
 }
 
-__global__ void k_reset_counter(uint32_t* counter) {
-	if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+__global__ void k_reset_counters(uint32_t* counters) {
+	if (blockIdx.x == 0 && threadIdx.x < kBatchFamilies) counters[threadIdx.x] = 0;
 }
 
 // Cauchy matrices of every job of a batch: thread -> (job, r, c) through the
@@ -398,17 +430,20 @@ __global__ void k_cauchy_batch(const BatchJob* __restrict__ jobs, uint32_t njobs
 }
 
 // XOR-block coefficients of every dyadic job of a batch: 8 threads per
-// (job, q, block) pair, job found by binary search over jobs[j].gp_off; the
-// same values as k_dyadic_coeffs.
+// 8-word unit, job found by binary search over jobs[j].gp_off; the same values
+// as k_dyadic_coeffs (bits 3: a unit is one (q, block) pair) and
+// k_dyadic16_coeffs (bits 4: units 2p, 2p+1 are pair p, from an even unit, so
+// a pair's 16 threads are 16-aligned in the work-group).
 __global__ __launch_bounds__(256) void k_dyadic_coeffs_batch(const BatchJob* __restrict__ jobs,
                                                              uint32_t njobs, uint64_t total) {
 	__shared__ uint64_t gs[256];
 	const int u = threadIdx.x & 7;
-	for (uint64_t base_pair = (uint64_t)blockIdx.x * 32; base_pair < total;
-	     base_pair += (uint64_t)gridDim.x * 32) {
-		const uint64_t idx = base_pair + (threadIdx.x >> 3);
+	for (uint64_t base_unit = (uint64_t)blockIdx.x * 32; base_unit < total;
+	     base_unit += (uint64_t)gridDim.x * 32) {
+		const uint64_t idx = base_unit + (threadIdx.x >> 3);
 		uint64_t c = 0;
 		uint32_t lo = 0;
+		bool b16 = false;
 		if (idx < total) {
 			uint32_t hi = njobs - 1;
 			while (lo < hi) {
@@ -417,16 +452,30 @@ __global__ __launch_bounds__(256) void k_dyadic_coeffs_batch(const BatchJob* __r
 			}
 			const BatchJob& jb = jobs[lo];
 			const uint64_t e = idx - jb.gp_off;
-			const uint64_t q = e / jb.nblocks, b = e - q * jb.nblocks;
-			const uint64_t z = (((jb.fi & ~7ull) + 8 * b) ^ ((jb.fr & ~7ull) + 8 * q)) ^ (uint64_t)u;
+			b16 = jb.bits == 4;
+			uint64_t z;
+			if (b16) {
+				const uint64_t pr = e >> 1, q = pr / jb.nblocks, b = pr - q * jb.nblocks;
+				z = (((jb.fi & ~15ull) + 16 * b) ^ ((jb.fr & ~15ull) + 16 * q)) ^ (uint64_t)(8 * (e & 1) + u);
+			} else {
+				const uint64_t q = e / jb.nblocks, b = e - q * jb.nblocks;
+				z = (((jb.fi & ~7ull) + 8 * b) ^ ((jb.fr & ~7ull) + 8 * q)) ^ (uint64_t)u;
+			}
 			c = z ? gf64_inv(z) : 1u;
 		}
 		gs[threadIdx.x] = c;
 		__syncthreads();
 		uint64_t acc = 0;
+		if (b16) {
+			const int uu = threadIdx.x & 15;
 #pragma unroll
-		for (int t = 0; t < 8; t++)
-			if ((t & u) == u) acc ^= gs[(threadIdx.x & ~7) + t];
+			for (int t = 0; t < 16; t++)
+				if ((t & uu) == uu) acc ^= gs[(threadIdx.x & ~15) + t];
+		} else {
+#pragma unroll
+			for (int t = 0; t < 8; t++)
+				if ((t & u) == u) acc ^= gs[(threadIdx.x & ~7) + t];
+		}
 		if (idx < total) {
 			const BatchJob& jb = jobs[lo];
 			jb.gp[(idx - jb.gp_off) * 8 + u] = acc;
@@ -584,7 +633,7 @@ hipError_t launch_xor_reduce(uint64_t* partial, uint64_t G, uint64_t R, uint64_t
 }
 
 hipError_t launch_batch(const BatchJob* jobs, uint32_t njobs, const BatchTile* tiles,
-                        uint32_t ntiles, uint32_t* counter, const uint32_t* zero_ids,
+                        const uint32_t* fam_tiles, uint32_t* counters, const uint32_t* zero_ids,
                         uint32_t nzero, uint64_t coeff_total, uint64_t dy_pairs, int blocks,
                         hipStream_t s) {
 	if (coeff_total) {
@@ -603,10 +652,19 @@ hipError_t launch_batch(const BatchJob* jobs, uint32_t njobs, const BatchTile* t
 		hipLaunchKernelGGL(k_zero_outputs, dim3(64, nzero < 1024 ? nzero : 1024), dim3(256), 0, s,
 		                   jobs, zero_ids, nzero);
 	}
+	uint32_t ntiles = 0;
+	for (int f = 0; f < kBatchFamilies && fam_tiles; f++) ntiles += fam_tiles[f];
 	if (ntiles && blocks > 0) {
-		hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, counter);
-		hipLaunchKernelGGL(k_batch_persistent, dim3((unsigned)blocks), dim3(256), 0, s, jobs, tiles,
-		                   ntiles, counter);
+		hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters);
+		uint32_t off = 0;
+		for (int f = 0; f < kBatchFamilies; f++) {
+			const uint32_t n = fam_tiles[f];
+			if (!n) continue;
+			void (*k)(const BatchJob*, const BatchTile*, uint32_t, uint32_t*) =
+			    f == 0 ? k_batch_persistent<0> : (f == 1 ? k_batch_persistent<1> : k_batch_persistent<2>);
+			hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, s, jobs, tiles + off, n, counters + f);
+			off += n;
+		}
 	}
 	return hipGetLastError();
 }

@@ -32,7 +32,8 @@ enum BatchCfg : uint16_t {
 	BATCH_CFG_4_4_4 = 4,   // RT=4,  WPT=4, IB=4
 	BATCH_CFG_DY4 = 5,     // XOR-block (dyadic) tiles, WPT=4 (full 1024-word tiles)
 	BATCH_CFG_DY2 = 6,     // XOR-block (dyadic) tiles, WPT=2 (full 512-word tiles)
-	BATCH_CFG_COUNT = 7
+	BATCH_CFG_DY16 = 7,    // 16 x 16 XOR-block tiles, WPT=1 (full 256-word tiles)
+	BATCH_CFG_COUNT = 8
 };
 struct BatchJob {
 	const uint64_t* in;
@@ -43,13 +44,18 @@ struct BatchJob {
 	uint64_t N, R, W;
 	uint64_t fi, fr;
 	uint64_t coeff_off;   // first global index of this job's coefficients (0 of them for dyadic jobs)
-	// dyadic jobs: Q x nblocks x 8 XOR-block coefficients (k_dyadic_coeffs layout)
+	// dyadic jobs: Q x nblocks x 2^bits XOR-block coefficients (k_dyadic_coeffs
+	// layout; bits 4: k_dyadic16_coeffs layout); nblocks, Q, pre_in, pre_out
+	// count in runs of 2^bits points
 	uint64_t* gp;
 	uint64_t nblocks, Q, pre_in, pre_out;
-	uint64_t gp_off;      // first global (q, block) pair of this job (Q*nblocks pairs; 0 otherwise)
+	// first global 8-word unit of this job's coefficients (Q*nblocks units, or
+	// 2*Q*nblocks from an even unit for bits 4; 0 for other jobs)
+	uint64_t gp_off;
+	uint64_t bits;        // 3: 8 x 8 blocks, 4: 16 x 16 blocks
 };
 // Table tiles: outputs [r0, r0+RT), inputs [i0, i1).  Dyadic tiles: output
-// group r0, input blocks [i0, i1).  Both: words [w0, w0 + 256*WPT).
+// group r0, input blocks [i0, i1) (of 8 or 16 points).  Both: words [w0, w0 + 256*WPT).
 struct BatchTile {
 	uint32_t job;
 	uint16_t cfg;         // BatchCfg
@@ -59,11 +65,20 @@ struct BatchTile {
 };
 
 // The batch's launch sequence: coefficient matrices (coeff_total words over
-// the table / MFMA jobs), XOR-block coefficients (dy_pairs (q, block) pairs
-// over the dyadic jobs), zeroing of the atomically accumulated jobs' outputs,
+// the table / MFMA jobs), XOR-block coefficients (dy_pairs 8-word units over
+// the dyadic jobs), zeroing of the atomically accumulated jobs' outputs,
 // queue reset, persistent kernel.
+// Tiles come grouped by family (table tiles, 8 x 8 XOR-block tiles, 16 x 16
+// XOR-block tiles; fam_tiles[f] of each, in that order), and each family runs
+// as its own persistent kernel over its own queue counter (counters[f]): one
+// kernel holding every variant would size its registers for the largest and
+// spill in the others.
+constexpr int kBatchFamilies = 3;
+inline int batch_family(int cfg) {
+	return cfg == BATCH_CFG_DY16 ? 2 : ((cfg == BATCH_CFG_DY4 || cfg == BATCH_CFG_DY2) ? 1 : 0);
+}
 hipError_t launch_batch(const BatchJob* jobs, uint32_t njobs, const BatchTile* tiles,
-                        uint32_t ntiles, uint32_t* counter, const uint32_t* zero_ids,
+                        const uint32_t* fam_tiles, uint32_t* counters, const uint32_t* zero_ids,
                         uint32_t nzero, uint64_t coeff_total, uint64_t dy_pairs, int blocks,
                         hipStream_t s);
 
