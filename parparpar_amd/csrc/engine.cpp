@@ -581,12 +581,13 @@ bool dyadic16_better(uint64_t R, uint64_t fr) {
 }
 
 DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev,
-                       size_t max_part_bytes, bool allow16) {
+                       size_t max_part_bytes, bool allow16, int force_bits) {
 	DyadicPlan p;
 	// Block size: 16 x 16 blocks take 81 products per 256 input-output pairs,
 	// 8 x 8 blocks 27 per 64; per input that is Q16 * 81/16 against
 	// Q8 * 27/8 (R = 16, 32, 64: a quarter less; R = 24: equal; R <= 8: 8 x 8).
 	if (allow16 && dyadic16_better(R, fr)) p.bits = 4;
+	if (force_bits == 3 || force_bits == 4) p.bits = force_bits;
 	const uint64_t m = p.bits == 4 ? 15 : 7;
 	p.pre_in = fi & m;
 	p.pre_out = fr & m;

@@ -68,8 +68,9 @@ bool dyadic16_better(uint64_t R, uint64_t fr);
 // allow16: the caller can run the 16 x 16 form (coefficients computed in the
 // kernel or in the 16-word layout, no lost masks); the plan takes it where it
 // needs fewer table reads.
+// force_bits 3 / 4: that block size whatever the rule says (workspace bounds).
 DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev,
-                       size_t max_part_bytes = SIZE_MAX, bool allow16 = false);
+                       size_t max_part_bytes = SIZE_MAX, bool allow16 = false, int force_bits = 0);
 // Whether `a` (strided slices, this thread's method) takes the dyadic kernel.
 bool dyadic_applies(const MacArgs& a);
 // Whether R outputs take it under this thread's method (any input layout);

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256, WPT == 2 ? 4 : 2) void k_mac_dyadic(DyadicArgs
 
 // The 16 x 16 form (gf64_dyadic16_dev.h): same grid, blocks and groups of 16
 // points; 58 KiB of LDS, two work-groups per CU.
-template <int WPT, bool FAST, bool GP>
+template <int WPT, bool FAST, bool GP, bool MASK = false>
 __global__ __launch_bounds__(256, 2) void k_mac_dyadic16(DyadicArgs a) {
 	__shared__ uint4 tab[2 * kDyTables * 256];   // two sets of seven pair tables
 	__shared__ uint64_t coef[kD16Chunk * 16];    // coefficients computed here (GP false)
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256, 2) void k_mac_dyadic16(DyadicArgs a) {
 	const uint64_t b_begin = g * a.blocks_per_group;
 	uint64_t b_end = b_begin + a.blocks_per_group;
 	if (b_end > a.nblocks) b_end = a.nblocks;
-	d16_tile<WPT, FAST, GP>(a, tab, coef, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end, g,
+	d16_tile<WPT, FAST, GP, MASK>(a, tab, coef, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end, g,
 	                        a.partial ? DY_OUT_PARTIAL : (a.atomic_out ? DY_OUT_ATOMIC : DY_OUT_DIRECT));
 }
 
@@ -86,9 +86,18 @@ hipError_t launch_mac_dyadic16(const DyadicArgs& a, uint64_t groups, int wpt, hi
 	const uint64_t tiles = (a.W + 256ull * wpt - 1) / (256ull * wpt);
 	if (!tiles || !groups || !Q) return hipSuccess;
 	if (tiles > 0x7fffffffu || groups > 65535 || Q > 65535) return hipErrorInvalidValue;
-	if (a.lost || (wpt != 1 && wpt != 2)) return hipErrorInvalidValue;
+	if ((wpt != 1 && wpt != 2) || (a.lost && !a.gp)) return hipErrorInvalidValue;
 	dim3 grid((unsigned)tiles, (unsigned)groups, (unsigned)Q);
 	const bool fast = a.W % (256ull * wpt) == 0;
+	if (a.lost) {
+		// lost-input masks (the repair's right-hand side; its coefficients
+		// always come from gp)
+		if (wpt == 2 && fast) hipLaunchKernelGGL((k_mac_dyadic16<2, true, true, true>), grid, dim3(256), 0, s, a);
+		else if (wpt == 2) hipLaunchKernelGGL((k_mac_dyadic16<2, false, true, true>), grid, dim3(256), 0, s, a);
+		else if (fast) hipLaunchKernelGGL((k_mac_dyadic16<1, true, true, true>), grid, dim3(256), 0, s, a);
+		else hipLaunchKernelGGL((k_mac_dyadic16<1, false, true, true>), grid, dim3(256), 0, s, a);
+		return hipGetLastError();
+	}
 	const int v = (wpt == 2 ? 4 : 0) | (fast ? 2 : 0) | (a.gp ? 1 : 0);
 	switch (v) {
 #define D16_CASE(V, W, F, G) \

@@ -21,7 +21,8 @@
 // inputs, outputs not stored), as in the 8 x 8 form.  The launch arguments are
 // the same DyadicArgs: nblocks, blocks_per_group, pre_in, pre_out count in
 // runs of 16; gp holds 16 words per (output group q, block b) at
-// gp + (q*gp_ld + b)*16.  No lost-input masks.
+// gp + (q*gp_ld + b)*16; lost masks (MASK launches) one u32 per block of 16
+// points, bit t = virtual input 16b + t counts as zero.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -152,19 +153,20 @@ __device__ __forceinline__ void d16_build(uint4* tab, const uint64_t (&c)[4]) {
 }
 
 // This thread's words w0 + tid + 256*j (j < WPT) of the 16 inputs of block b,
-// zeros where the input is virtual or past the group's blocks, or the word is
-// past the slice end (clamped indices, selects: straight-line loads).  FAST:
-// full tiles, no word checks.
+// zeros where the input is virtual, past the group's blocks or flagged in
+// `lost` (loaded like the others and zeroed: its memory may hold anything), or
+// the word is past the slice end (clamped indices, selects: straight-line
+// loads).  FAST: full tiles, no word checks.
 template <int WPT, bool FAST>
 __device__ __forceinline__ void d16_load(const DyadicArgs& a, uint64_t b, uint64_t b_end, uint64_t w0,
-                                         uint2 (&d)[16][WPT]) {
+                                         uint32_t lost, uint2 (&d)[16][WPT]) {
 	const uint64_t wl = w0 + threadIdx.x;
 #pragma unroll
 	for (int t = 0; t < 16; t++) {
 		const uint64_t iv = 16 * b + t - a.pre_in;  // wraps for a virtual point in front
 		const bool valid = (b < b_end) & (iv < a.N);
 		const uint64_t i = valid ? iv : 0;
-		const uint32_t on = valid ? ~0u : 0u;
+		const uint32_t on = (valid & !((lost >> t) & 1u)) ? ~0u : 0u;
 		const uint64_t* src = a.in_base + i * a.in_stride;
 #pragma unroll
 		for (int j = 0; j < WPT; j++) {
@@ -247,7 +249,7 @@ __device__ __forceinline__ void d16_word(const uint4* tab, uint2 (&x)[16], uint2
 // One tile: words [w0, w0 + 256*WPT) of output group q (virtual recovery
 // points 16q..16q+15), blocks [b_begin, b_end); tab: 2 x 7 x 256 uint4 of LDS;
 // coef: kD16Chunk x 16 words of LDS (GP false).  Output modes as dy_tile.
-template <int WPT, bool FAST, bool GP>
+template <int WPT, bool FAST, bool GP, bool MASK = false>
 __device__ __forceinline__ void d16_tile(const DyadicArgs& a, uint4* tab, uint64_t* coef, uint64_t q,
                                          uint64_t w0, uint64_t b_begin, uint64_t b_end, uint64_t g,
                                          int out_mode) {
@@ -263,7 +265,7 @@ __device__ __forceinline__ void d16_tile(const DyadicArgs& a, uint4* tab, uint64
 		__syncthreads();
 	}
 	d16_load_coeffs<GP>(a, coef, q, b_begin, b_begin, b_end, c);
-	d16_load<WPT, FAST>(a, b_begin, b_end, w0, d);
+	d16_load<WPT, FAST>(a, b_begin, b_end, w0, dy_lost<MASK>(a, b_begin, b_end), d);
 	for (uint64_t bb = b_begin; bb < b_end; bb++) {
 		__syncthreads();
 		d16_build(tab, c);
@@ -275,6 +277,7 @@ __device__ __forceinline__ void d16_tile(const DyadicArgs& a, uint4* tab, uint64
 			__syncthreads();
 		}
 		d16_load_coeffs<GP>(a, coef, q, bb + 1, b_begin, b_end, c);
+		const uint32_t lost_next = dy_lost<MASK>(a, bb + 1, b_end);
 #pragma unroll
 		for (int j = 0; j < WPT; j++) {
 			uint2 x[16];
@@ -283,7 +286,7 @@ __device__ __forceinline__ void d16_tile(const DyadicArgs& a, uint4* tab, uint64
 			d16_word(tab, x, acc[j]);
 			// the next block's words j (the registers are free now)
 			uint2 dn[16][1];
-			d16_load<1, FAST>(a, bb + 1, b_end, w0 + 256 * j, dn);
+			d16_load<1, FAST>(a, bb + 1, b_end, w0 + 256 * j, lost_next, dn);
 #pragma unroll
 			for (int t = 0; t < 16; t++) d[t][j] = dn[t][0];
 		}

@@ -119,7 +119,7 @@ hipError_t launch_dyadic_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uin
 bool mac_dyadic_supported(int wpt, int ib);
 hipError_t launch_mac_dyadic(const DyadicArgs& a, uint64_t groups, int wpt, int ib, hipStream_t s);
 // The 16 x 16 form: blocks and output groups of 16 points, gp with 16 words
-// per (q, b); wpt 1 or 2; no lost masks.
+// per (q, b), lost masks one u32 per block of 16 (with gp only); wpt 1 or 2.
 hipError_t launch_dyadic16_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uint64_t xbase,
                                   uint64_t ybase, hipStream_t s);
 hipError_t launch_mac_dyadic16(const DyadicArgs& a, uint64_t groups, int wpt, hipStream_t s);

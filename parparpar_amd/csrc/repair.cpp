@@ -62,7 +62,7 @@ struct RepairArchive {
 struct RepairPlan {
 	std::vector<uint64_t> host_idx;  // table route: [xs_known | xs_missing | ys]; else [xs_missing | ys]
 	std::vector<const uint64_t*> in_ptrs;
-	std::vector<uint32_t> host_mask;  // XOR-block route: lost inputs per block of 8 points
+	std::vector<uint32_t> host_mask;  // XOR-block route: lost inputs per block of 8 or 16 points
 	std::vector<uint64_t> host_inv;   // A^-1 from the host (colliding points only)
 	void* ws = nullptr;
 	size_t ws_bytes = 0;
@@ -93,7 +93,7 @@ static size_t repair_ws_bytes(uint64_t N, uint64_t W, uint64_t k, const DyadicPl
 	size_t b = 0;
 	if (dp) {
 		b += align256(dp->nblocks * 4);             // lost-input masks
-		b += align256(dp->nblocks * dp->Q * 64);    // XOR-block coefficients
+		b += align256(dp->nblocks * dp->Q * 8 * (dp->bits == 4 ? 16 : 8));  // XOR-block coefficients
 		b += align256(dp->part_bytes);              // split-K partials of the RHS
 		b += align256(k * 8) * 2;                   // xs_missing, ys
 	} else {
@@ -110,12 +110,13 @@ static size_t repair_ws_bytes(uint64_t N, uint64_t W, uint64_t k, const DyadicPl
 
 // Device bytes a repair of the first k recovery rows needs beyond its output:
 // workspace + the split-K partials of its contractions.
+// force_bits: an XOR-block size whatever the rule picks (0: the rule's).
 static size_t repair_bytes(uint64_t N, uint64_t W, uint64_t k, uint64_t fi, uint64_t fr,
-                           bool auto_method) {
+                           bool auto_method, int force_bits = 0) {
 	const int dev = current_device();
 	const size_t solve = plan_workspace_bytes(plan_contraction(k, k, W, dev), k, W);
 	if (repair_dyadic(nullptr, k, auto_method)) {
-		const DyadicPlan dp = plan_dyadic(N, k, W, fi, fr, dev);
+		const DyadicPlan dp = plan_dyadic(N, k, W, fi, fr, dev, SIZE_MAX, true, force_bits);
 		return repair_ws_bytes(N, W, k, &dp) + solve;
 	}
 	return repair_ws_bytes(N, W, k, nullptr) +
@@ -186,7 +187,8 @@ static int repair_enqueue(const RepairArchive& ar, const uint64_t* missing, uint
 	}
 	const uint64_t yfirst = ar.fr + (rows ? rows[0] : 0);
 	DyadicPlan dp{};
-	if (dy) dp = plan_dyadic(N, k, W, ar.fi, yfirst, current_device());
+	if (dy) dp = plan_dyadic(N, k, W, ar.fi, yfirst, current_device(), SIZE_MAX, true);
+	const uint64_t bm = dp.bits == 4 ? 15 : 7;  // points per XOR block - 1
 	// workspace carve-up
 	plan.ws_bytes = repair_ws_bytes(N, W, k, dy ? &dp : nullptr);
 	int rc = dev_alloc(&plan.ws, plan.ws_bytes, s);
@@ -199,14 +201,14 @@ static int repair_enqueue(const RepairArchive& ar, const uint64_t* missing, uint
 	hipError_t e;
 	if (dy) {
 		d_mask = (uint32_t*)p;      p += align256(dp.nblocks * 4);
-		d_gp = (uint64_t*)p;        p += align256(dp.nblocks * dp.Q * 64);
+		d_gp = (uint64_t*)p;        p += align256(dp.nblocks * dp.Q * 8 * (bm + 1));
 		d_part = (uint64_t*)p;      p += align256(dp.part_bytes);
 		d_xm = (uint64_t*)p;        p += align256(k * 8) * 2;
 		d_ys = d_xm + k;  // uploaded together with xs_missing
 		plan.host_mask.assign(dp.nblocks, 0u);
 		for (uint64_t m = 0; m < k; m++) {
 			const uint64_t v = dp.pre_in + missing[m];  // virtual slot of the lost input
-			plan.host_mask[v / 8] |= 1u << (v % 8);
+			plan.host_mask[v / (bm + 1)] |= 1u << (v % (bm + 1));
 		}
 		e = hipMemcpyAsync(d_mask, plan.host_mask.data(), dp.nblocks * 4, hipMemcpyHostToDevice, s);
 		if (e == hipSuccess) e = hipMemcpyAsync(d_xm, xs_miss, 2 * k * 8, hipMemcpyHostToDevice, s);
@@ -237,7 +239,8 @@ static int repair_enqueue(const RepairArchive& ar, const uint64_t* missing, uint
 	if (dy) {
 		// RHS = the rows used, XORed with the contraction of every input but
 		// the lost ones (masked to zero) at the rows' points
-		e = launch_dyadic_coeffs(d_gp, dp.nblocks, dp.Q, ar.fi & ~7ull, yfirst & ~7ull, s);
+		e = dp.bits == 4 ? launch_dyadic16_coeffs(d_gp, dp.nblocks, dp.Q, ar.fi & ~bm, yfirst & ~bm, s)
+		                 : launch_dyadic_coeffs(d_gp, dp.nblocks, dp.Q, ar.fi & ~bm, yfirst & ~bm, s);
 		if (e == hipSuccess)
 			e = hipMemcpy2DAsync(d_rhs, W * 8, ar.rec + (yfirst - ar.fr) * ar.rec_stride,
 			                     ar.rec_stride * 8, W * 8, k, hipMemcpyDeviceToDevice, s);
@@ -537,9 +540,11 @@ int parhip_d_gf64_repair(const uint64_t* d_data, size_t numInputs, size_t dataSt
 }
 
 size_t parhip_gf64_repair_workspace_bytes(size_t numInputs, size_t blockSize64, size_t k) {
-	// points in the worst alignment (7 virtual slots before the first input
-	// and the first recovery point): an upper bound for any archive
-	return repair_bytes(numInputs, blockSize64, k, 7, 7, false);
+	// points in the worst alignment of either XOR-block size (7 or 15
+	// virtual slots before the first input and the first recovery point): an
+	// upper bound for any archive
+	return std::max(repair_bytes(numInputs, blockSize64, k, 7, 7, false, 3),
+	                repair_bytes(numInputs, blockSize64, k, 15, 15, false, 4));
 }
 
 parhip_repair_server* parhip_repair_server_create(size_t hbmBudgetBytes, unsigned maxInflight,

@@ -227,8 +227,9 @@ def test_server_colliding_points(gpu):
 
 
 # The two RHS routes: the create's XOR-block contraction over the whole archive
-# with the lost inputs masked to zero (consecutive rows, k >= 4) and the
-# explicit-coefficient tables (forced with the table method).  Points start
+# with the lost inputs masked to zero (consecutive rows, k >= 4; blocks of 8
+# or 16 points) and the explicit-coefficient tables (forced with the table
+# method).  Points start
 # off a multiple of 8, so virtual slots precede the first input and the first
 # recovery point; lost inputs fall in one block of 8, across blocks and at the
 # ends; the lost slots hold junk.
@@ -239,6 +240,10 @@ ROUTE_CASES = [
     (64, 2048, 16, 0, 64, [7, 8, 15, 16, 63], None),
     (1000, 96, 64, 1, 1001, list(range(0, 1000, 17))[:58], None),
     (40, 1024, 8, 7, 47, [0, 1, 2, 3, 4, 5, 6, 39], None),
+    # 16 x 16 blocks: 13 virtual slots in front, lost inputs inside one block
+    # of 16 and across blocks, k = 16 and 33
+    (500, 1024, 48, 13, 13 + 500 + 3, list(range(2, 18)), list(range(5, 21))),
+    (333, 520, 40, 29, 29 + 333, [0, 1, 2] + list(range(100, 130)), None),
 ]
 
 
@@ -259,14 +264,14 @@ def test_repair_routes_agree(gpu, case):
                                 device="cuda", generator=g)
     outs = []
     try:
-        for m in (P.METHOD_AUTO, P.METHOD_TABLE):
+        for m in (P.METHOD_AUTO, P.METHOD_DYADIC8, P.METHOD_DYADIC16, P.METHOD_TABLE):
             P.set_method(m)
             outs.append(repair.repair(damaged, rec, missing, first_input=fi, first_recovery=fr, rows=rows))
     finally:
         P.set_method(P.METHOD_AUTO)
     torch.cuda.synchronize()
-    assert torch.equal(outs[0], data[tm])
-    assert torch.equal(outs[1], data[tm])
+    for o in outs:
+        assert torch.equal(o, data[tm])
 
 
 def test_server_inverse_cache_hits_and_eviction(gpu):
