@@ -110,7 +110,10 @@ static int run_gf16_mfma(const Gf16Args& m, bool fp4, hipStream_t s) {
 	if (fp4) groups = std::max<uint64_t>(groups, (m.N + (1u << 19) - 1) >> 19);
 	groups = std::min<uint64_t>(groups, 65535);
 	uint64_t per = (m.N + groups - 1) / groups;
-	per = (per + 3) / 4 * 4;  // stages of 4 inputs, pairs never straddle groups
+	// stages of 4 * ib inputs (ib = 4 unless tuned to 2; 4 inputs per MFMA K
+	// step), pairs never straddle groups
+	const int gib = fp4 && tib == 2 ? 2 : 4;
+	per = (per + 4 * gib - 1) / (4 * gib) * (4 * gib);
 	groups = (m.N + per - 1) / per;
 	if (fp4 && per >= (1u << 20))
 		return fail(PARHIP_ERR_ARG, "gf16 mfma: %llu inputs per group exceed the exact fp32 count",
@@ -140,7 +143,7 @@ static int run_gf16_mfma(const Gf16Args& m, bool fp4, hipStream_t s) {
 	a.accumulate = m.accumulate;
 	if (e == hipSuccess) {
 		ProfScope ps("gf16_mfma", s);
-		e = launch_gf16_mfma(a, groups, ot, nb, fp4, s);
+		e = launch_gf16_mfma(a, groups, ot, nb, gib, fp4, s);
 	}
 	if (e == hipSuccess && groups > 1) {
 		ProfScope ps("gf16_reduce", s);

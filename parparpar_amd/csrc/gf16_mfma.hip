@@ -299,6 +299,246 @@ __global__ __launch_bounds__(256, OCC) void k_gf16_mfma(Gf16MfmaArgs a) {
 	}
 }
 
+// The fp4 kernel for full tiles (W a multiple of 256 words, 4-byte aligned
+// rows at an even stride, 32-bit offsets; launch_gf16_mfma checks them): the
+// same tiles, K order and staging as k_gf16_mfma<4, 2, 2, fp4, 2>, with the
+// address work taken out of the loop.  A stage's input rows and row words are
+// a uniform (scalar) base plus a per-lane offset fixed for the whole launch,
+// and a lane loads only the two inputs of its B operand (lane half h: inputs
+// 2h, 2h+1 of every 4), one 32-bit load each, so a B operand is one byte
+// permute plus the fp4 expansion.  Stages that run past the input range (the
+// last one) take the clamped form.
+template <int IB>
+__global__ __launch_bounds__(256, 2) void k_gf16_fp4f(Gf16MfmaArgs a) {
+	constexpr int OT = 4, NB = 2;
+	constexpr int LPT = IB * OT * 64 / 256;  // row words staged per thread per stage
+	static_assert((IB * OT * 64) % 256 == 0, "whole waves stage row words");
+	__shared__ uint4 sA[2][IB][OT][64];
+
+	const int tid = threadIdx.x;
+	const int lane = tid & 63;
+	const int wave = tid >> 6;
+	const int col = lane & 31;
+	const int h = lane >> 5;
+	const uint64_t Z = (a.R + 2 * OT - 1) / (2 * OT);
+	const uint64_t L = blockIdx.x;
+	const uint64_t rem = L % (8 * Z);
+	const uint64_t xblk = (L / (8 * Z)) * 8 + (rem & 7);
+	if (xblk * 4 * 32 * NB >= a.W) return;  // padding of the last run (whole work-group)
+	const uint64_t w0 = (xblk * 4 + wave) * (32ull * NB) + (uint64_t)NB * col;
+	const uint64_t r0 = (rem >> 3) * (2 * OT);
+	const uint64_t g = blockIdx.y;
+	const uint64_t i_begin = g * a.in_per_group;
+	uint64_t i_end = i_begin + a.in_per_group;
+	if (i_end > a.N) i_end = a.N;
+	const uint64_t P = (a.N + 1) / 2;
+
+	// per-lane offsets, fixed for the launch: input 2h (+ in_stride for 2h+1)
+	// word w0, in bytes; row words (e - r0, pair pr, bit o) in words
+	const uint32_t xoff = (uint32_t)(((uint64_t)(2 * h) * a.in_stride + w0) * 2);
+	const uint32_t xstep = (uint32_t)(a.in_stride * 2);
+	uint32_t roff[LPT];
+#pragma unroll
+	for (int k = 0; k < LPT; k++) {
+		const int j = tid + 256 * k;  // (ib, t, pair, row)
+		const int ib = j / (OT * 64), t = (j >> 6) % OT, rho = j & 31, pr = (j >> 5) & 1;
+		uint64_t e = r0 + 2 * t + (rho >> 4);
+		e = e < a.R ? e : a.R - 1;
+		roff[k] = (uint32_t)(((e - r0) * P + 2 * ib + pr) * 16 + (rho & 15));
+	}
+
+	g16_v16f acc[OT][NB];
+#pragma unroll
+	for (int t = 0; t < OT; t++)
+#pragma unroll
+		for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+			for (int e = 0; e < 16; e++) acc[t][nb][e] = 0;
+
+	// two stages of loads in flight: stage s computes from registers loaded
+	// two stages before (set s & 1), while the loads of stage s + 2 refill
+	// that set and stage s + 1's rows, loaded a stage earlier, are staged
+	uint32_t pre2[2][LPT];     // row words
+	uint32_t xp2[2][2 * IB];   // this lane's input words
+	uint32_t pre[LPT];
+	uint32_t xc[2 * IB];
+
+	// a full stage: every input of [ibase, ibase + 4 IB) is in range
+	auto load_full = [&](uint64_t ibase, uint32_t (&pr)[LPT], uint32_t (&xp)[2 * IB]) {
+		const uint32_t* rbase = a.rows + (r0 * P + ibase / 2) * 16;
+		const char* xbase = reinterpret_cast<const char*>(a.in_base + ibase * a.in_stride);
+#pragma unroll
+		for (int k = 0; k < LPT; k++) pr[k] = rbase[roff[k]];
+#pragma unroll
+		for (int ib = 0; ib < IB; ib++) {
+			const char* xb = xbase + (uint64_t)(4 * ib) * a.in_stride * 2;
+			xp[2 * ib] = *reinterpret_cast<const uint32_t*>(xb + xoff);
+			xp[2 * ib + 1] = *reinterpret_cast<const uint32_t*>(xb + xoff + xstep);
+		}
+	};
+	auto store_stage = [&](int buf, uint64_t ibase, bool full, const uint32_t (&rw)[LPT]) {
+#pragma unroll
+		for (int k = 0; k < LPT; k++) {
+			const int j = tid + 256 * k;
+			const int ib = j / (OT * 64), t = (j >> 6) % OT, rho = j & 31, pr = (j >> 5) & 1;
+			uint32_t wv = rw[k];
+			if (!full) {
+				const uint64_t i0 = ibase + 4 * ib + 2 * pr;  // inputs of this pair: i0, i0 + 1
+				wv = (i0 < i_end ? wv & 0xFFFFu : 0u) | (i0 + 1 < i_end ? wv & 0xFFFF0000u : 0u);
+			}
+			sA[buf][ib][t][rho + 32 * pr] = g16_expand_fp4(wv);
+		}
+	};
+	// B operands of K step ib: word nb of inputs 2h, 2h+1 is one byte
+	// permute plus the fp4 expansion
+	auto bops = [&](int ib, g16_v8i (&bf)[NB]) {
+#pragma unroll
+		for (int nb = 0; nb < NB; nb++) {
+			const uint32_t pw = __builtin_amdgcn_perm(xc[2 * ib + 1], xc[2 * ib], nb ? 0x07060302u : 0x05040100u);
+			const uint4 ex = g16_expand_fp4(pw);
+			bf[nb][0] = (int)ex.x;
+			bf[nb][1] = (int)ex.y;
+			bf[nb][2] = (int)ex.z;
+			bf[nb][3] = (int)ex.w;
+#pragma unroll
+			for (int v = 4; v < 8; v++) bf[nb][v] = 0;
+		}
+	};
+	// K step ib's MFMAs with step ib + 1's B operands computed between them:
+	// the expansion (VALU) runs while the matrix core works on this step
+	// (sched_group_barrier: each MFMA followed by two VALU ops)
+	auto compute = [&](int buf) {
+		g16_v8i bfa[NB], bfb[NB];
+		bops(0, bfa);
+#pragma unroll
+		for (int ib = 0; ib < IB; ib++) {
+			g16_v8i(&cur)[NB] = (ib & 1) ? bfb : bfa;
+			g16_v8i(&nxt)[NB] = (ib & 1) ? bfa : bfb;
+			uint4 av[OT];
+#pragma unroll
+			for (int t = 0; t < OT; t++) av[t] = sA[buf][ib][t][lane];
+			if (ib + 1 < IB) bops(ib + 1 < IB ? ib + 1 : ib, nxt);
+#pragma unroll
+			for (int t = 0; t < OT; t++) {
+				g16_v8i af;
+				af[0] = (int)av[t].x;
+				af[1] = (int)av[t].y;
+				af[2] = (int)av[t].z;
+				af[3] = (int)av[t].w;
+#pragma unroll
+				for (int v = 4; v < 8; v++) af[v] = 0;
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++)
+					acc[t][nb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(af, cur[nb], acc[t][nb], 4, 4, 0, 128,
+					                                                             0, 128);
+			}
+			__builtin_amdgcn_sched_group_barrier(0x100, OT, 0);  // this step's A reads first
+#pragma unroll
+			for (int k = 0; k < OT * NB; k++) {
+				__builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+				__builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // two VALU
+			}
+			__builtin_amdgcn_sched_barrier(0);
+		}
+	};
+
+	// full stages in the loop; a last stage that runs past the input range
+	// after it, with clamped loads (its rows past the range staged as zeros)
+	const uint64_t full_end = i_begin + (i_end > i_begin ? (i_end - i_begin) / (4 * IB) * (4 * IB) : 0);
+	int buf = 0;
+	if (full_end > i_begin) {  // uniform over the work-group
+		constexpr uint64_t SP = 4 * IB;  // inputs per stage
+		// a stage past the last full one loads the last one again (harmless:
+		// its staging is never read), so the body has no branch for the
+		// compiler to sink the MFMAs past
+		auto clampst = [&](uint64_t ib) { return ib < full_end ? ib : full_end - SP; };
+		load_full(i_begin, pre2[0], xp2[0]);
+		load_full(clampst(i_begin + SP), pre2[1], xp2[1]);
+		store_stage(0, i_begin, true, pre2[0]);
+		__syncthreads();
+		// one stage with its loads in set S: compute from set S, refill S
+		// with stage + 2, stage stage + 1 from set S ^ 1
+		auto step = [&](uint64_t ibase, auto S) {
+			constexpr int cs = decltype(S)::value;
+#pragma unroll
+			for (int u = 0; u < 2 * IB; u++) xc[u] = xp2[cs][u];
+			load_full(clampst(ibase + 2 * SP), pre2[cs], xp2[cs]);
+			compute(buf);
+			store_stage(buf ^ 1, clampst(ibase + SP), true, pre2[cs ^ 1]);
+			__syncthreads();
+			buf ^= 1;
+		};
+		for (uint64_t ibase = i_begin; ibase < full_end; ibase += 2 * SP) {
+			step(ibase, std::integral_constant<int, 0>());
+			if (ibase + SP >= full_end) break;
+			step(ibase + SP, std::integral_constant<int, 1>());
+		}
+	}
+	if (full_end < i_end) {
+		const uint64_t ibase = full_end;
+		const uint64_t i_last = i_end - 1;
+#pragma unroll
+		for (int k = 0; k < LPT; k++) {
+			const int j = tid + 256 * k;
+			const int ib = j / (OT * 64), t = (j >> 6) % OT, rho = j & 31, pr = (j >> 5) & 1;
+			uint64_t e = r0 + 2 * t + (rho >> 4);
+			e = e < a.R ? e : a.R - 1;
+			uint64_t ip = ibase / 2 + 2 * ib + pr;
+			ip = ip < P ? ip : P - 1;
+			pre[k] = a.rows[(e * P + ip) * 16 + (rho & 15)];
+		}
+#pragma unroll
+		for (int ib = 0; ib < IB; ib++)
+#pragma unroll
+			for (int v = 0; v < 2; v++) {
+				uint64_t i = ibase + 4 * ib + 2 * h + v;
+				i = i < i_end ? i : i_last;
+				xc[2 * ib + v] = *reinterpret_cast<const uint32_t*>(a.in_base + i * a.in_stride + w0);
+			}
+		store_stage(buf, ibase, false, pre);
+		__syncthreads();
+		compute(buf);
+	}
+	// parity -> 16-bit words (as k_gf16_mfma): lane (col, h) holds rows
+	// 8(j/4) + 4h + (j%4) of each 32-row tile; after the exchange with lane
+	// col + 32, lane half h writes output 2t + h, words w0, w0 + 1.
+#pragma unroll
+	for (int t = 0; t < OT; t++) {
+		uint32_t ov = 0;
+#pragma unroll
+		for (int nb = 0; nb < NB; nb++) {
+			uint32_t b0 = 0, b1 = 0;
+#pragma unroll
+			for (int j = 0; j < 8; j++) {
+				const int o = 8 * (j >> 2) + 4 * h + (j & 3);
+				b0 |= ((uint32_t)acc[t][nb][j] & 1u) << o;
+				b1 |= ((uint32_t)acc[t][nb][j + 8] & 1u) << o;
+			}
+			b0 |= (uint32_t)__shfl_xor((int)b0, 32);
+			b1 |= (uint32_t)__shfl_xor((int)b1, 32);
+			ov |= (h ? b1 : b0) << (16 * nb);
+		}
+		const uint64_t e = r0 + 2 * t + h;
+		if (e >= a.R) continue;
+		uint16_t* dst;
+		bool xr;
+		if (a.partial) {
+			dst = a.partial + (g * a.R + e) * a.W;
+			xr = false;
+		} else {
+			dst = a.out_base + e * a.out_stride;
+			xr = a.accumulate != 0;
+		}
+		if ((reinterpret_cast<uintptr_t>(dst + w0) & 3u) == 0) {
+			uint32_t* q = reinterpret_cast<uint32_t*>(dst + w0);
+			*q = xr ? (*q ^ ov) : ov;
+		} else {
+			dst[w0] = xr ? (uint16_t)(dst[w0] ^ ov) : (uint16_t)ov;
+			dst[w0 + 1] = xr ? (uint16_t)(dst[w0 + 1] ^ (ov >> 16)) : (uint16_t)(ov >> 16);
+		}
+	}
+}
+
 hipError_t launch_gf16_rows(const uint16_t* coeff, uint64_t ld, uint64_t N, uint64_t R, uint32_t* rows,
                             hipStream_t s) {
 	const uint64_t total = R * ((N + 1) / 2) * 16;
@@ -325,8 +565,29 @@ bool gf16_mfma_supported(int ot, int nb, bool /*fp4*/) {
 	return (ot == 4 && (nb == 4 || nb == 2)) || (ot == 8 && nb == 2);
 }
 
-hipError_t launch_gf16_mfma(const Gf16MfmaArgs& a, uint64_t groups, int ot, int nb, bool fp4,
+// Whether the full-tile fp4 kernel takes the launch: 256-word tiles, 4-byte
+// aligned input rows at an even stride, the per-lane offsets in 32 bits, and
+// stages of 8 inputs that never straddle split-K rows.
+static bool fp4f_ok(const Gf16MfmaArgs& a) {
+	const uint64_t P = (a.N + 1) / 2;
+	return a.W % 256 == 0 && (reinterpret_cast<uintptr_t>(a.in_base) & 3u) == 0 && a.in_stride % 2 == 0 &&
+	       (3 * a.in_stride + a.W) * 2 < (1ull << 32) && 8 * P * 16 + 64 < (1ull << 32) &&
+	       a.in_per_group % 8 == 0;
+}
+
+hipError_t launch_gf16_mfma(const Gf16MfmaArgs& a, uint64_t groups, int ot, int nb, int ib, bool fp4,
                             hipStream_t s) {
+	if (fp4 && ot == 4 && nb == 2 && fp4f_ok(a) && (ib == 2 || ib == 4)) {
+		const uint64_t span = 4ull * 32 * 2;
+		const uint64_t nx = (a.W + span - 1) / span, Z = (a.R + 7) / 8;
+		const uint64_t blocks = (nx + 7) / 8 * 8 * Z;
+		if (blocks > 0x7fffffffu || groups > 65535) return hipErrorInvalidValue;
+		if (ib == 4 && a.in_per_group % 16 == 0)
+			hipLaunchKernelGGL((k_gf16_fp4f<4>), dim3((unsigned)blocks, (unsigned)groups, 1), dim3(256), 0, s, a);
+		else
+			hipLaunchKernelGGL((k_gf16_fp4f<2>), dim3((unsigned)blocks, (unsigned)groups, 1), dim3(256), 0, s, a);
+		return hipGetLastError();
+	}
 	if (ot == 4 && nb == 4) return fp4 ? launch_g16m_t<4, 4, 4, true, 1>(a, groups, s)
 	                                   : launch_g16m_t<4, 4, 4, false, 1>(a, groups, s);
 	if (ot == 4 && nb == 2) return fp4 ? launch_g16m_t<4, 2, 2, true, 2>(a, groups, s)

@@ -165,7 +165,7 @@ struct Gf16MfmaArgs {
 hipError_t launch_gf16_rows(const uint16_t* coeff, uint64_t ld, uint64_t N, uint64_t R, uint32_t* rows,
                             hipStream_t s);
 bool gf16_mfma_supported(int ot, int nb, bool fp4);
-hipError_t launch_gf16_mfma(const Gf16MfmaArgs& a, uint64_t groups, int ot, int nb, bool fp4,
+hipError_t launch_gf16_mfma(const Gf16MfmaArgs& a, uint64_t groups, int ot, int nb, int ib, bool fp4,
                             hipStream_t s);
 
 hipError_t launch_gf16_coeffs(uint16_t* coeff, const uint16_t* input_nums, uint64_t N,

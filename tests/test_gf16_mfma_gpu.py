@@ -41,7 +41,7 @@ def _case(torch, n, w, r, seed):
 @pytest.mark.parametrize("m", METHODS)
 @pytest.mark.parametrize("n,w,r,acc", [(1, 1, 1, False), (2, 3, 2, True), (5, 33, 7, False),
                                        (16, 1000, 9, True), (37, 2049, 17, False), (64, 512, 64, True),
-                                       (130, 700, 40, False)])
+                                       (130, 700, 40, False), (37, 256, 17, False), (203, 1024, 45, True)])
 def test_small_shapes(use_method, oracle, m, n, w, r, acc):
     import torch
     from parparpar_amd import device
@@ -74,6 +74,25 @@ def test_split_k_and_tile_shapes(use_method, m):
         out = device.gf16_muladd(x, co)
         torch.cuda.synchronize()
         assert torch.equal(out, ref), (m, ot, nb, groups)
+    pp.check(pp.lib().parhip_tune_override(0, 0, 0, 0))
+
+
+def test_fp4_full_tile_kernel_shapes(use_method):
+    """The full-tile fp4 kernel (slices of a multiple of 256 words): stages of
+    8 and 16 inputs, split-K rows, a ragged last stage (203 inputs), output
+    tiles past R (45), against the split-table kernel."""
+    import torch
+    import parparpar_amd as pp
+    from parparpar_amd import device
+    x, inums, exps, co = _case(torch, 203, 2048, 45, 11)
+    ref = device.gf16_muladd(x, co)  # split tables (AUTO)
+    torch.cuda.synchronize()
+    use_method("fp4")
+    for (ib, groups) in [(0, 1), (0, 3), (4, 1), (4, 5), (2, 17)]:
+        pp.check(pp.lib().parhip_tune_override(0, 0, ib, groups))
+        out = device.gf16_muladd(x, co)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), (ib, groups)
     pp.check(pp.lib().parhip_tune_override(0, 0, 0, 0))
 
 
