@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--slice-bytes", type=int, default=1 << 20)
     ap.add_argument("--recovery", type=int, default=8)
     ap.add_argument("--method", default="auto",
-                    choices=["auto", "dyadic", "table", "table8", "mfma", "mfma-i8"],
+                    choices=["auto", "dyadic", "dyadic8", "dyadic16", "table", "table8", "mfma", "mfma-i8"],
                     help="contraction strategy: auto (the XOR-block kernel for plain Cauchy "
                          "calls), nibble or byte tables in LDS, or the GF(2) bit-matrix on "
                          "fp4 / int8 MFMA")
@@ -225,12 +225,17 @@ def main():
     return main_par3(args)
 
 
-METHODS = {"auto": 0, "table": 1, "mfma": 2, "mfma-i8": 3, "table8": 4, "dyadic": 5}
+METHODS = {"auto": 0, "table": 1, "mfma": 2, "mfma-i8": 3, "table8": 4, "dyadic": 5, "dyadic8": 6,
+           "dyadic16": 7}
 MAC_NAME = {"mfma": "mac_mfma", "mfma-i8": "mac_mfma", "table8": "mac_tab8", "table": "mac_tab",
-            "auto": "mac_dyadic", "dyadic": "mac_dyadic"}
+            "auto": "mac_dyadic", "dyadic": "mac_dyadic", "dyadic8": "mac_dyadic",
+            "dyadic16": "mac_dyadic16"}
 METHOD_TEXT = {
-    "auto": "XOR-block (dyadic) Cauchy contraction: 8x8 blocks as subset convolutions, 27 "
-            "products per block, nibble tables in LDS (k_mac_dyadic) + split-K XOR reduce",
+    "auto": "XOR-block (dyadic) Cauchy contraction: 8x8 blocks as subset convolutions (27 products "
+            "per block; 16x16 blocks, 81 products, where they need fewer table reads), nibble tables "
+            "in LDS read one table ahead (k_mac_dyadic / k_mac_dyadic16) + split-K XOR reduce",
+    "dyadic8": "XOR-block Cauchy contraction, 8x8 blocks only (k_mac_dyadic) + split-K XOR reduce",
+    "dyadic16": "XOR-block Cauchy contraction, 16x16 blocks (k_mac_dyadic16) + split-K XOR reduce",
     "table8": "8-bit split tables in LDS (k_mac_tab8) + split-K XOR reduce",
     "mfma": "GF(2) bit-matrix on fp4 MFMA (k_mac_mfma)",
     "mfma-i8": "GF(2) bit-matrix on int8 MFMA, int32 parity (k_mac_mfma)",
@@ -262,11 +267,23 @@ def measure_par3(args, d_in, n, sb, r, steps, warmup):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    names = ("mac_dyadic", "mac_tab", "mac_tab8", "mac_mfma", "mfma_rows", "xor_reduce",
-             "dyadic_coeffs", "cauchy")
+    names = ("mac_dyadic", "mac_dyadic16", "mac_tab", "mac_tab8", "mac_mfma", "mfma_rows", "xor_reduce",
+             "zero_rows", "dyadic_coeffs", "cauchy")
+    # the contraction kernel this shape runs (the automatic choice may take
+    # 16 x 16 XOR blocks): one probed step
+    mac = MAC_NAME[args.method]
+    P.prof_only(None)
+    P.prof_enable(True)
+    P.prof_reset()
+    step()
+    torch.cuda.synchronize()
+    P.prof_enable(False)
+    for k in ("mac_dyadic16", "mac_dyadic", "mac_tab", "mac_tab8", "mac_mfma"):
+        if P.prof_get(k)[1]:
+            mac = k
+            break
     # the timed steps: HIP events around the contraction kernel only (two per
     # step; events around every kernel add ~20 us to a 0.54 ms step)
-    mac = MAC_NAME[args.method]
     P.prof_only(mac)
     P.prof_enable(True)
     P.prof_reset()
@@ -293,7 +310,7 @@ def measure_par3(args, d_in, n, sb, r, steps, warmup):
         ms, cnt = P.prof_get(k)
         if cnt and k not in kern:
             kern[k] = ms / cnt
-    return n * sb * steps / 2 ** 30 / elapsed, elapsed / steps * 1e3, kern, d_out
+    return n * sb * steps / 2 ** 30 / elapsed, elapsed / steps * 1e3, kern, d_out, mac
 
 
 def main_par3(args):
@@ -310,7 +327,6 @@ def main_par3(args):
                          generator=gen)
     P.set_method(METHODS[args.method])
     P.check(L.parhip_set_coeff_cache(0))  # every step builds its own coefficients
-    mac_name = MAC_NAME[args.method]
     if args.copy_calibration_mib:
         words = args.copy_calibration_mib * (1 << 20) // 8
         src = torch.ones(words, dtype=torch.int64, device="cuda")
@@ -319,7 +335,7 @@ def main_par3(args):
         torch.add(src, 0, out=dst)  # reads and writes exactly words*8 bytes
         torch.cuda.synchronize()
         del src, dst
-    value, ms_step, kern, d_out = measure_par3(args, d_in, n, sb, r, args.steps, args.warmup)
+    value, ms_step, kern, d_out, mac_name = measure_par3(args, d_in, n, sb, r, args.steps, args.warmup)
     sp = torch.cuda.current_stream().cuda_stream
     # the step after create: R sealed PAR REC packets from the recovery blocks in HBM
     d_pk = torch.empty((r, 64 + sb), dtype=torch.uint8, device="cuda")
@@ -396,7 +412,7 @@ def main_par3(args):
             line["cpu_baseline"] = {"error": repr(e)}
     if headline and not args.headline_only:
         # the rest of the metric, each in its own bracket of this run
-        v2, ms2, k2, o2 = measure_par3(args, d_in.view(262144, 512), 262144, 4096, 8, 50, 50)
+        v2, ms2, k2, o2, _ = measure_par3(args, d_in.view(262144, 512), 262144, 4096, 8, 50, 50)
         del o2
         line["par3_262144x4KiB_r8"] = {
             "GiBps": round(v2, 3), "ms_per_step": round(ms2, 4),
@@ -460,7 +476,8 @@ def measure_par2(args, x, steps=10, warmup=20, cpu=True):
     res = {
         "GiBps": round(n * sb * steps / 2 ** 30 / el, 3),
         "ms_per_step": round(el / steps * 1e3, 3),
-        "method": {"mfma": "GF(2) bit-matrix on fp4 MFMA (k_gf16_rows + k_gf16_mfma)",
+        "method": {"mfma": "GF(2) bit-matrix on fp4 MFMA (k_gf16_rows + k_gf16_fp4f for full 256-word "
+                           "tiles, else k_gf16_mfma)",
                    "mfma-i8": "GF(2) bit-matrix on int8 MFMA (k_gf16_rows + k_gf16_mfma)"}.get(
             args.method, "GF(2^16) nibble split tables in LDS (k_gf16_tab) + split-K XOR reduce"),
         "kernels_ms_avg": {kname: round(tab_ms / max(tab_n, 1), 3),
@@ -662,9 +679,12 @@ def measure_batch(args):
     out = {
         "workload": "batched uneven PAR3 create: %d jobs, %.2f GiB total, size LogUniform[4 MiB, 512 MiB], "
                     "slice {4,8,32,1024} KiB, R U{4..64}" % (len(jobs), gib),
-        "method": "persistent k_batch_persistent + device tile queue + hipGraph; autotuned per shape: "
-                  "table launch variants, or the GF(2) bit-matrix on MFMA where eligible",
-        "plain": "one parhip_d_gf64_recovery call per job (XOR-block kernel where R >= 4)",
+        "method": "persistent kernels, one per tile family (tables, 8x8 and 16x16 XOR blocks), over "
+                  "device tile queues, in one hipGraph; per shape: XOR-block tiles (block size by "
+                  "table reads, tile width autotuned), or the GF(2) bit-matrix on MFMA where it wins",
+        "plain": "one parhip_d_gf64_recovery call per job (the same XOR-block kernels where R >= 4)",
+        "jobs_16x16_blocks": sum(b.job_config(k)[0] == 16 and b.job_method(k) == P.METHOD_DYADIC
+                                 for k in range(len(jobs))),
         "steps": steps,
         "plain_GiBps": round(gib / res["plain"], 3),
         "batched_GiBps": round(gib / res["batched"], 3),
