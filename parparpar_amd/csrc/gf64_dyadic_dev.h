@@ -378,6 +378,40 @@ __device__ __forceinline__ void dy_pair_step(const DyadicArgs& a, const uint4* t
 	dy_load_pair<WPT, JJ, FAST>(a, bb + 1, b_end, w0, live_next, d);
 }
 
+// Table build of one block from the coefficients c0, c1 of this thread's pair
+// table: thread (tbl, half, n) owns entry n of 8 of the 16 positions of one
+// pair table; entry(k, n) = x^4 * entry(k-1, n), the second half starting from
+// x^32 * entry(0, n).  The 16 lanes of a (table, half) write 16 consecutive
+// entries of one row per store.
+__device__ __forceinline__ void dy_build(uint4* tab, uint64_t c0, uint64_t c1) {
+	const int tid = threadIdx.x;
+	if (tid < kDyTables * 32) {
+		// thread (tbl, half, n): entry n of positions 8*half .. 8*half+7
+		const int n = tid & 15;
+		const int half = (tid >> 4) & 1;
+		const int tbl = tid >> 5;
+		uint64_t e0 = gf64_mul_nibble(c0, (unsigned)n);
+		uint64_t e1 = gf64_mul_nibble(c1, (unsigned)n);
+		if (half) {
+			e0 = gf64_mulx32(e0);
+			e1 = gf64_mulx32(e1);
+		}
+#pragma unroll
+		for (int k = 0; k < 8; k++) {
+			if (k) {
+				e0 = gf64_mulx4(e0);
+				e1 = gf64_mulx4(e1);
+			}
+			uint4 v;
+			v.x = (uint32_t)e0;
+			v.y = (uint32_t)(e0 >> 32);
+			v.z = (uint32_t)e1;
+			v.w = (uint32_t)(e1 >> 32);
+			tab[(tbl * 16 + 8 * half + k) * 16 + n] = v;
+		}
+	}
+}
+
 // One tile: words [w0, w0 + 256*WPT) of output group q (virtual recovery
 // points 8q..8q+7), input blocks [b_begin, b_end); `tab` is 7 x 256 uint4 of
 // LDS.  out_mode: DY_OUT_DIRECT stores (or XORs when a.accumulate) into the
@@ -394,6 +428,8 @@ enum { DY_OUT_DIRECT = 0, DY_OUT_PARTIAL = 1, DY_OUT_ATOMIC = 2 };
 // Measured on 1024 x 1 MiB: 4 % fewer cycles at R = 8; at R >= 16 the cycle
 // count is the same and the clock lower (the chip's power limit), so the plan
 // takes it for one output group only.
+// (Two table buffers, block bb + 1 built beside block bb's reads with one
+// barrier per block, measured 3-5 % slower at 1024 x 1 MiB, R = 8 and 24.)
 template <int WPT, bool FAST, bool MASK, bool GP, bool PIPE = false>
 __device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_t* coef, uint64_t q,
                                         uint64_t w0, uint64_t b_begin, uint64_t b_end, uint64_t g,
@@ -426,31 +462,7 @@ __device__ __forceinline__ void dy_tile(const DyadicArgs& a, uint4* tab, uint64_
 		// second half starting from x^32 * entry(0, n).  The 16 lanes of a
 		// (table, half) write 16 consecutive entries of one row per store.
 		__syncthreads();
-		if (tid < kDyTables * 32) {
-			// thread (tbl, half, n): entry n of positions 8*half .. 8*half+7
-			const int n = tid & 15;
-			const int half = (tid >> 4) & 1;
-			const int tbl = tid >> 5;
-			uint64_t e0 = gf64_mul_nibble(c0, (unsigned)n);
-			uint64_t e1 = gf64_mul_nibble(c1, (unsigned)n);
-			if (half) {
-				e0 = gf64_mulx32(e0);
-				e1 = gf64_mulx32(e1);
-			}
-#pragma unroll
-			for (int k = 0; k < 8; k++) {
-				if (k) {
-					e0 = gf64_mulx4(e0);
-					e1 = gf64_mulx4(e1);
-				}
-				uint4 v;
-				v.x = (uint32_t)e0;
-				v.y = (uint32_t)(e0 >> 32);
-				v.z = (uint32_t)e1;
-				v.w = (uint32_t)(e1 >> 32);
-				tab[(tbl * 16 + 8 * half + k) * 16 + n] = v;
-			}
-		}
+		dy_build(tab, c0, c1);
 		__syncthreads();
 		if (!GP && bb + 1 < b_end && (bb + 1 - b_begin) % kDyChunk == 0) {
 			// (every read of the previous chunk is done: its last values were
