@@ -194,6 +194,27 @@ def cpu_baseline_par2(x_dev, words, r, budget_s):
     }
 
 
+def measured_copy_GBps(mib=1024, reps=10):
+    """Device-to-device copy of `mib` MiB (read + write), best of `reps`, GB/s."""
+    import torch
+    words = mib * (1 << 20) // 8
+    src = torch.ones(words, dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        b.synchronize()
+        best = min(best, a.elapsed_time(b) / 1e3)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * words * 8 / best / 1e9
+
+
 def read_profile_value(name, key):
     """A figure recorded under profiles/ (rocprofv3 counter runs of this kernel)."""
     path = os.path.join(ROOT, "profiles", name)
@@ -390,6 +411,12 @@ def main_par3(args):
         },
         "cpu_baseline": None,
     }
+    if headline:
+        # a STREAM-like copy on the same device (1 GiB read + 1 GiB written,
+        # HIP events on torch's stream): the HBM rate a plain kernel reaches here
+        peak_meas = measured_copy_GBps()
+        line["roofline"]["measured_copy_GBps"] = round(peak_meas, 1)
+        line["roofline"]["frac_of_measured_copy"] = round(achieved / peak_meas, 4)
     if mac_name == "mac_dyadic" and headline:
         # the kernel's own limit: LDS delivery of the nibble-table reads
         busy = read_profile_value("mac_dyadic_1024x1MiB_r8.txt", "lds_busy")
