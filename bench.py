@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--slice-bytes", type=int, default=1 << 20)
     ap.add_argument("--recovery", type=int, default=8)
     ap.add_argument("--method", default="auto",
-                    choices=["auto", "dyadic", "dyadic8", "dyadic16", "table", "table8", "mfma", "mfma-i8"],
+                    choices=["auto", "dyadic", "dyadic8", "dyadic16", "dyadic32", "table", "table8", "mfma",
+                             "mfma-i8"],
                     help="contraction strategy: auto (the XOR-block kernel for plain Cauchy "
                          "calls), nibble or byte tables in LDS, or the GF(2) bit-matrix on "
                          "fp4 / int8 MFMA")
@@ -247,16 +248,18 @@ def main():
 
 
 METHODS = {"auto": 0, "table": 1, "mfma": 2, "mfma-i8": 3, "table8": 4, "dyadic": 5, "dyadic8": 6,
-           "dyadic16": 7}
+           "dyadic16": 7, "dyadic32": 8}
 MAC_NAME = {"mfma": "mac_mfma", "mfma-i8": "mac_mfma", "table8": "mac_tab8", "table": "mac_tab",
             "auto": "mac_dyadic", "dyadic": "mac_dyadic", "dyadic8": "mac_dyadic",
-            "dyadic16": "mac_dyadic16"}
+            "dyadic16": "mac_dyadic16", "dyadic32": "mac_dyadic32"}
 METHOD_TEXT = {
     "auto": "XOR-block (dyadic) Cauchy contraction: 8x8 blocks as subset convolutions (27 products "
-            "per block; 16x16 blocks, 81 products, where they need fewer table reads), nibble tables "
-            "in LDS read one table ahead (k_mac_dyadic / k_mac_dyadic16) + split-K XOR reduce",
+            "per block; 16x16 blocks, 81 products, or 32x32 blocks, 243 products, where they need "
+            "fewer table reads), nibble tables in LDS read one table ahead (k_mac_dyadic / "
+            "k_mac_dyadic16 / k_mac_dyadic32) + split-K XOR reduce",
     "dyadic8": "XOR-block Cauchy contraction, 8x8 blocks only (k_mac_dyadic) + split-K XOR reduce",
     "dyadic16": "XOR-block Cauchy contraction, 16x16 blocks (k_mac_dyadic16) + split-K XOR reduce",
+    "dyadic32": "XOR-block Cauchy contraction, 32x32 blocks (k_mac_dyadic32) + split-K XOR reduce",
     "table8": "8-bit split tables in LDS (k_mac_tab8) + split-K XOR reduce",
     "mfma": "GF(2) bit-matrix on fp4 MFMA (k_mac_mfma)",
     "mfma-i8": "GF(2) bit-matrix on int8 MFMA, int32 parity (k_mac_mfma)",
@@ -288,10 +291,10 @@ def measure_par3(args, d_in, n, sb, r, steps, warmup):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    names = ("mac_dyadic", "mac_dyadic16", "mac_tab", "mac_tab8", "mac_mfma", "mfma_rows", "xor_reduce",
-             "zero_rows", "dyadic_coeffs", "cauchy")
+    names = ("mac_dyadic", "mac_dyadic16", "mac_dyadic32", "mac_tab", "mac_tab8", "mac_mfma", "mfma_rows",
+             "xor_reduce", "zero_rows", "dyadic_coeffs", "cauchy")
     # the contraction kernel this shape runs (the automatic choice may take
-    # 16 x 16 XOR blocks): one probed step
+    # 16 x 16 or 32 x 32 XOR blocks): one probed step
     mac = MAC_NAME[args.method]
     P.prof_only(None)
     P.prof_enable(True)
@@ -299,7 +302,7 @@ def measure_par3(args, d_in, n, sb, r, steps, warmup):
     step()
     torch.cuda.synchronize()
     P.prof_enable(False)
-    for k in ("mac_dyadic16", "mac_dyadic", "mac_tab", "mac_tab8", "mac_mfma"):
+    for k in ("mac_dyadic32", "mac_dyadic16", "mac_dyadic", "mac_tab", "mac_tab8", "mac_mfma"):
         if P.prof_get(k)[1]:
             mac = k
             break

@@ -47,11 +47,14 @@ extern "C" {
 #define PARHIP_METHOD_MFMA 2     /* GF(2) bit-matrix on fp4 MFMA (exact 0/1 counts) */
 #define PARHIP_METHOD_MFMA_I8 3  /* GF(2) bit-matrix on int8 MFMA (int32 parity)    */
 #define PARHIP_METHOD_TABLE8 4   /* 8-bit split tables in LDS (256-entry rows)      */
-#define PARHIP_METHOD_DYADIC 5   /* plain Cauchy calls: 8 x 8 or 16 x 16 XOR blocks */
-                                 /* as subset convolutions (27 / 81 products per    */
-                                 /* block), nibble tables in LDS; others as AUTO    */
+#define PARHIP_METHOD_DYADIC 5   /* plain Cauchy calls: 8 x 8, 16 x 16 or 32 x 32   */
+                                 /* XOR blocks as subset convolutions (27 / 81 /    */
+                                 /* 243 products per block), nibble tables in LDS;  */
+                                 /* others as AUTO                                  */
 #define PARHIP_METHOD_DYADIC8 6  /* as DYADIC with 8 x 8 blocks only                */
 #define PARHIP_METHOD_DYADIC16 7 /* as DYADIC with 16 x 16 blocks wherever the call */
+                                 /* allows them (plain Cauchy create)               */
+#define PARHIP_METHOD_DYADIC32 8 /* as DYADIC with 32 x 32 blocks wherever the call */
                                  /* allows them (plain Cauchy create)               */
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -146,6 +146,7 @@ METHOD_TABLE8 = 4
 METHOD_DYADIC = 5
 METHOD_DYADIC8 = 6
 METHOD_DYADIC16 = 7
+METHOD_DYADIC32 = 8
 
 
 class ParhipError(RuntimeError):

@@ -150,9 +150,9 @@ static const size_t kCoeffCacheMax = 16;
 
 // Words of the dyadic coefficients for (N, R, fi, fr).
 // kind 1: XOR-block coefficients of 8 x 8 blocks (8 words per (q, b));
-// kind 2: of 16 x 16 blocks (16 words per (q, b)).
+// kind 2: of 16 x 16 blocks (16 words per (q, b)); kind 3: of 32 x 32 blocks.
 static uint64_t dyadic_coeff_words(int kind, uint64_t n, uint64_t r, uint64_t fi, uint64_t fr) {
-	const uint64_t m = kind == 2 ? 15 : 7;
+	const uint64_t m = (8ull << (kind - 1)) - 1;
 	const uint64_t nblocks = ((fi & m) + n + m) / (m + 1), Q = ((fr & m) + r + m) / (m + 1);
 	return nblocks * Q * (m + 1);
 }
@@ -168,6 +168,11 @@ int coeff_acquire(uint64_t n, uint64_t r, uint64_t fi, uint64_t fr, hipStream_t 
 // Build a coefficient set of `kind` into `d` on stream s.
 static hipError_t coeff_build(int kind, uint64_t* d, uint64_t n, uint64_t r, uint64_t fi,
                               uint64_t fr, hipStream_t s) {
+	if (kind == 3) {
+		ProfScope ps("dyadic_coeffs", s);
+		return launch_dyadic32_coeffs(d, ((fi & 31) + n + 31) / 32, ((fr & 31) + r + 31) / 32,
+		                              fi & ~31ull, fr & ~31ull, s);
+	}
 	if (kind == 2) {
 		ProfScope ps("dyadic_coeffs", s);
 		return launch_dyadic16_coeffs(d, ((fi & 15) + n + 15) / 16, ((fr & 15) + r + 15) / 16,
@@ -573,28 +578,45 @@ int run_contraction_tables(MacArgs a, hipStream_t s) {
 // Split-K rows from which the XOR-block launches fold with atomics.
 static const uint64_t kDyAtomicRows = 16;
 
-bool dyadic16_better(uint64_t R, uint64_t fr) {
-	if (t_method == PARHIP_METHOD_DYADIC8) return false;
-	if (t_method == PARHIP_METHOD_DYADIC16) return true;
-	const uint64_t q8 = ((fr & 7) + R + 7) / 8, q16 = ((fr & 15) + R + 15) / 16;
-	return (double)q16 * 81 / 16 < 0.97 * (double)q8 * 27 / 8;
+int dyadic_block_bits(uint64_t R, uint64_t fr, int max_bits) {
+	if (t_method == PARHIP_METHOD_DYADIC8) return 3;
+	if (t_method == PARHIP_METHOD_DYADIC16) return std::min(4, max_bits);
+	if (t_method == PARHIP_METHOD_DYADIC32) return std::min(5, max_bits);
+	// table reads per input: Q_b output groups x 3^b products per 2^b inputs;
+	// a larger block where it needs at least 3 % fewer
+	int bits = 3;
+	double best = (double)(((fr & 7) + R + 7) / 8) * 27 / 8;
+	for (int b = 4; b <= max_bits; b++) {
+		const uint64_t m = (1ull << b) - 1;
+		const double cost = (double)(((fr & m) + R + m) >> b) * (b == 4 ? 81.0 / 16 : 243.0 / 32);
+		if (cost < 0.97 * best) {
+			best = cost;
+			bits = b;
+		}
+	}
+	return bits;
 }
 
+bool dyadic16_better(uint64_t R, uint64_t fr) { return dyadic_block_bits(R, fr, 4) == 4; }
+
 DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev,
-                       size_t max_part_bytes, bool allow16, int force_bits) {
+                       size_t max_part_bytes, bool allow16, int force_bits, bool allow32) {
 	DyadicPlan p;
-	// Block size: 16 x 16 blocks take 81 products per 256 input-output pairs,
-	// 8 x 8 blocks 27 per 64; per input that is Q16 * 81/16 against
-	// Q8 * 27/8 (R = 16, 32, 64: a quarter less; R = 24: equal; R <= 8: 8 x 8).
-	if (allow16 && dyadic16_better(R, fr)) p.bits = 4;
-	if (force_bits == 3 || force_bits == 4) p.bits = force_bits;
-	const uint64_t m = p.bits == 4 ? 15 : 7;
+	// Block size: 32 x 32 blocks take 243 products per 1024 input-output
+	// pairs, 16 x 16 blocks 81 per 256, 8 x 8 blocks 27 per 64; per input that
+	// is Q32 * 243/32, Q16 * 81/16 and Q8 * 27/8 (R <= 8: 8 x 8; R = 9..16:
+	// 16 x 16; R = 17..32: 32 x 32, a quarter less than 16 x 16 blocks).
+	if (allow16) p.bits = dyadic_block_bits(R, fr, allow32 ? 5 : 4);
+	if (force_bits >= 3 && force_bits <= 5) p.bits = force_bits;
+	const uint64_t m = (1ull << p.bits) - 1;
 	p.pre_in = fi & m;
 	p.pre_out = fr & m;
 	p.nblocks = (p.pre_in + N + m) / (m + 1);
 	p.Q = (p.pre_out + R + m) / (m + 1);
 	p.ib = 1;
-	if (p.bits == 4) {
+	if (p.bits == 5) {
+		p.wpt = 1;  // 512 threads, one word each
+	} else if (p.bits == 4) {
 		p.wpt = 1;
 		if (t_tune_wpt == 1 || t_tune_wpt == 2) p.wpt = t_tune_wpt;
 	} else {
@@ -607,7 +629,8 @@ DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t
 			}
 		}
 	}
-	const uint64_t tiles = (W + 256ull * p.wpt - 1) / (256ull * p.wpt);
+	const uint64_t tile_words = p.bits == 5 ? 512 : 256ull * p.wpt;
+	const uint64_t tiles = (W + tile_words - 1) / tile_words;
 	const uint64_t base = tiles * p.Q;
 	// Resident work-groups: 2 per CU at 4 words per thread (205 VGPRs), 4 at
 	// 2 words per thread (127 VGPRs, 28 KiB of LDS each; 262144 x 4 KiB,
@@ -616,13 +639,15 @@ DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t
 	// (whole rounds: a round that is half full costs a full one), plus the
 	// reduce, which reads g partial rows: 1024 x 1 MiB at R = 24 goes from
 	// 2 rows (768 work-groups, 1.5 rounds: 1.95 ms) to 4 (3 rounds: 1.54 ms).
-	// (16 x 16 blocks: 58 KiB of LDS, two work-groups per CU at either wpt)
-	const uint64_t slots = (uint64_t)device_cus(dev) * (p.bits == 3 && p.wpt == 2 ? 4 : 2);
+	// (16 x 16 blocks: 58 KiB of LDS, two work-groups per CU at either wpt;
+	// 32 x 32 blocks: 116 KiB, one work-group of 512 threads)
+	const uint64_t slots =
+	    (uint64_t)device_cus(dev) * (p.bits == 5 ? 1 : (p.bits == 3 && p.wpt == 2 ? 4 : 2));
 	const uint64_t gmax = std::min<uint64_t>(std::max<uint64_t>(1, p.nblocks / (2 * p.ib)), 65535);
 	// a slot's block step is ~16 us of 1024-word tile work (8 x 8 blocks; a
 	// 16 x 16 block of a 256-word tile is 3/4 of that per word); the reduce
 	// reads R x W words per row at ~4 TB/s
-	const double step_s = p.bits == 4 ? 16e-6 * 0.75 * p.wpt : 16e-6;
+	const double step_s = p.bits == 5 ? 16e-6 * 4.5 : p.bits == 4 ? 16e-6 * 0.75 * p.wpt : 16e-6;
 	const double row_cost = (double)R * W * 8 / 4e12 / step_s;
 	uint64_t groups = 1;
 	double best = 0;
@@ -668,7 +693,8 @@ bool dyadic_shape_ok(uint64_t R) {
 
 bool dyadic_applies_shape(uint64_t R) {
 	if (t_method != PARHIP_METHOD_AUTO && t_method != PARHIP_METHOD_DYADIC &&
-	    t_method != PARHIP_METHOD_DYADIC8 && t_method != PARHIP_METHOD_DYADIC16)
+	    t_method != PARHIP_METHOD_DYADIC8 && t_method != PARHIP_METHOD_DYADIC16 &&
+	    t_method != PARHIP_METHOD_DYADIC32)
 		return false;
 	return dyadic_shape_ok(R);
 }
@@ -710,7 +736,10 @@ hipError_t enqueue_dyadic(const MacArgs& a, const DyadicPlan& p, const uint64_t*
 		ProfScope ps("zero_rows", s);
 		e = launch_zero_rows(a.out_base, a.R, a.W, a.out_stride, s);
 	}
-	if (e == hipSuccess && p.bits == 4) {
+	if (e == hipSuccess && p.bits == 5) {
+		ProfScope ps("mac_dyadic32", s);
+		e = launch_mac_dyadic32(d, p.groups, s);
+	} else if (e == hipSuccess && p.bits == 4) {
 		ProfScope ps("mac_dyadic16", s);
 		e = launch_mac_dyadic16(d, p.groups, p.wpt, s);
 	} else if (e == hipSuccess) {
@@ -737,15 +766,15 @@ int run_cauchy_contraction(MacArgs a, uint64_t fi, uint64_t fr, hipStream_t s, u
 	if (dyadic_applies(a)) {
 		// the chunk's own virtual blocks start at block b0 of the call's
 		const uint64_t cfi = fi + first;
-		const DyadicPlan p = plan_dyadic(a.N, a.R, a.W, cfi, fr, t_device, SIZE_MAX, true);
-		const uint64_t m = p.bits == 4 ? 15 : 7;
+		const DyadicPlan p = plan_dyadic(a.N, a.R, a.W, cfi, fr, t_device, SIZE_MAX, true, 0, true);
+		const uint64_t m = (1ull << p.bits) - 1;
 		const uint64_t b0 = ((cfi & ~m) - (fi & ~m)) / (m + 1);
 		// cached coefficients where the cache is on; else the kernel computes
 		// them itself (one inversion per thread per chunk of blocks: no
 		// separate launch, no coefficient memory)
 		const uint64_t* gp = nullptr;
 		if (t_coeff_cache) {
-			int rc = coeff_acquire_kind(p.bits == 4 ? 2 : 1, n_total, a.R, fi, fr, s, &gp);
+			int rc = coeff_acquire_kind(p.bits - 2, n_total, a.R, fi, fr, s, &gp);
 			if (rc) return rc;
 		}
 		void* ws = nullptr;
@@ -759,7 +788,9 @@ int run_cauchy_contraction(MacArgs a, uint64_t fi, uint64_t fr, hipStream_t s, u
 		                                    nullptr, cfi & ~m, fr & ~m);
 		dev_free(ws, s);
 		if (gp) coeff_release(gp, s);
-		return e == hipSuccess ? PARHIP_OK : hip_fail(e, p.bits == 4 ? "k_mac_dyadic16" : "k_mac_dyadic");
+		return e == hipSuccess ? PARHIP_OK
+		                       : hip_fail(e, p.bits == 5 ? "k_mac_dyadic32"
+		                                                 : p.bits == 4 ? "k_mac_dyadic16" : "k_mac_dyadic");
 	}
 	const uint64_t* coeff = nullptr;
 	int rc = coeff_acquire(n_total, a.R, fi, fr, s, &coeff);
@@ -791,7 +822,7 @@ int parhip_set_device(int device) {
 }
 
 int parhip_set_method(int method) {
-	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_DYADIC16)
+	if (method < PARHIP_METHOD_AUTO || method > PARHIP_METHOD_DYADIC32)
 		return fail(PARHIP_ERR_ARG, "parhip_set_method: unknown method %d", method);
 	t_method = method;
 	return PARHIP_OK;

@@ -52,7 +52,8 @@ hipError_t enqueue_mfma(const MacArgs& m, const MfmaPlan& p, void* ws, hipStream
 
 // The XOR-block (dyadic) form of the plain Cauchy contraction (gf64_dyadic.hip).
 struct DyadicPlan {
-	// bits 3: 8 x 8 blocks (k_mac_dyadic); 4: 16 x 16 blocks (k_mac_dyadic16).
+	// bits 3: 8 x 8 blocks (k_mac_dyadic); 4: 16 x 16 blocks (k_mac_dyadic16);
+	// 5: 32 x 32 blocks (k_mac_dyadic32, tiles of 512 words).
 	// pre_in, pre_out, nblocks, Q count in runs of 2^bits points.
 	int bits = 3;
 	uint64_t pre_in, pre_out, nblocks, Q;
@@ -65,12 +66,19 @@ struct DyadicPlan {
 // Whether 16 x 16 XOR blocks need fewer table reads than 8 x 8 ones for R
 // outputs from point fr (or this thread's method forces either).
 bool dyadic16_better(uint64_t R, uint64_t fr);
+// The block size (log2 points: 3, 4 or 5, at most max_bits) that needs the
+// fewest table reads for R outputs from point fr, or the one this thread's
+// method forces.
+int dyadic_block_bits(uint64_t R, uint64_t fr, int max_bits);
 // allow16: the caller can run the 16 x 16 form (coefficients computed in the
-// kernel or in the 16-word layout, no lost masks); the plan takes it where it
-// needs fewer table reads.
-// force_bits 3 / 4: that block size whatever the rule says (workspace bounds).
+// kernel or in the 16-word layout); allow32 (with allow16): also the 32 x 32
+// form (coefficients in the kernel or in the 32-word layout); the plan takes
+// the block size that needs the fewest table reads.
+// force_bits 3 / 4 / 5: that block size whatever the rule says (workspace
+// bounds).
 DyadicPlan plan_dyadic(uint64_t N, uint64_t R, uint64_t W, uint64_t fi, uint64_t fr, int dev,
-                       size_t max_part_bytes = SIZE_MAX, bool allow16 = false, int force_bits = 0);
+                       size_t max_part_bytes = SIZE_MAX, bool allow16 = false, int force_bits = 0,
+                       bool allow32 = false);
 // Whether `a` (strided slices, this thread's method) takes the dyadic kernel.
 bool dyadic_applies(const MacArgs& a);
 // Whether R outputs take it under this thread's method (any input layout);

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "gf64_dyadic16_dev.h"
+#include "gf64_dyadic32_dev.h"
 #include "gf64_dyadic_dev.h"
 
 namespace parhip {
@@ -39,6 +40,80 @@ __global__ __launch_bounds__(256, 2) void k_mac_dyadic16(DyadicArgs a) {
 	if (b_end > a.nblocks) b_end = a.nblocks;
 	d16_tile<WPT, FAST, GP, MASK>(a, tab, coef, blockIdx.z, (uint64_t)blockIdx.x * 256 * WPT, b_begin, b_end, g,
 	                        a.partial ? DY_OUT_PARTIAL : (a.atomic_out ? DY_OUT_ATOMIC : DY_OUT_DIRECT));
+}
+
+// The 32 x 32 form (gf64_dyadic32_dev.h): tiles of 512 words, one work-group
+// of 512 threads per CU (116 KiB of LDS).
+template <bool FAST, bool GP, bool MASK>
+__global__ __launch_bounds__(kD32Threads, 1) void k_mac_dyadic32(DyadicArgs a) {
+	__shared__ uint4 tab[4 * kDyTables * 256];  // four sets of seven pair tables
+	__shared__ uint64_t coef[kD32Chunk * 32];   // coefficients computed here (GP false)
+	const uint64_t g = blockIdx.y;
+	const uint64_t b_begin = g * a.blocks_per_group;
+	uint64_t b_end = b_begin + a.blocks_per_group;
+	if (b_end > a.nblocks) b_end = a.nblocks;
+	d32_tile<FAST, GP, MASK>(a, tab, coef, blockIdx.z, (uint64_t)blockIdx.x * kD32Threads, b_begin, b_end, g,
+	                         a.partial ? DY_OUT_PARTIAL : (a.atomic_out ? DY_OUT_ATOMIC : DY_OUT_DIRECT));
+}
+
+// Transformed coefficients of every (output group q, input block b) of the
+// 32 x 32 form: g(u) = f(X_b XOR Y_q XOR u), X_b = xbase + 32b,
+// Y_q = ybase + 32q, out[(q*nblocks + b)*32 + S] = XOR over supersets T of S
+// of g(T); 8 (q, b) pairs per work-group.
+__global__ __launch_bounds__(256) void k_dyadic32_coeffs(uint64_t* __restrict__ out, uint64_t nblocks,
+                                                         uint64_t Q, uint64_t xbase, uint64_t ybase) {
+	const uint64_t total = nblocks * Q;
+	const int u = threadIdx.x & 31;
+	for (uint64_t base_pair = (uint64_t)blockIdx.x * 8; base_pair < total; base_pair += (uint64_t)gridDim.x * 8) {
+		const uint64_t idx = base_pair + (threadIdx.x >> 5);
+		uint64_t c = 0;
+		if (idx < total) {
+			const uint64_t q = idx / nblocks, b = idx - q * nblocks;
+			const uint64_t z = ((xbase + 32 * b) ^ (ybase + 32 * q)) ^ (uint64_t)u;
+			c = z ? gf64_inv(z) : 1u;  // the reference's 0 -> 1 (src/par3_engine.cc:392-394)
+		}
+#pragma unroll
+		for (int bit = 1; bit < 32; bit <<= 1) {
+			const uint32_t lo = __shfl_xor((uint32_t)c, bit, 64);
+			const uint32_t hi = __shfl_xor((uint32_t)(c >> 32), bit, 64);
+			if (!(u & bit)) c ^= ((uint64_t)hi << 32) | lo;
+		}
+		if (idx < total) out[idx * 32 + u] = c;
+	}
+}
+
+hipError_t launch_dyadic32_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uint64_t xbase,
+                                  uint64_t ybase, hipStream_t s) {
+	const uint64_t total = nblocks * Q;
+	if (!total) return hipSuccess;
+	uint64_t blocks = (total + 7) / 8;
+	if (blocks > 65536) blocks = 65536;
+	hipLaunchKernelGGL(k_dyadic32_coeffs, dim3((unsigned)blocks), dim3(256), 0, s, out, nblocks, Q, xbase,
+	                   ybase);
+	return hipGetLastError();
+}
+
+hipError_t launch_mac_dyadic32(const DyadicArgs& a, uint64_t groups, hipStream_t s) {
+	const uint64_t Q = (a.pre_out + a.R + 31) / 32;
+	const uint64_t tiles = (a.W + kD32Threads - 1) / kD32Threads;
+	if (!tiles || !groups || !Q) return hipSuccess;
+	if (tiles > 0x7fffffffu || groups > 65535 || Q > 65535) return hipErrorInvalidValue;
+	if (a.lost && !a.gp) return hipErrorInvalidValue;
+	dim3 grid((unsigned)tiles, (unsigned)groups, (unsigned)Q);
+	const bool fast = a.W % kD32Threads == 0;
+	const int v = (a.lost ? 4 : 0) | (fast ? 2 : 0) | (a.gp ? 1 : 0);
+	switch (v) {
+#define D32_CASE(V, F, G, M) \
+	case V: hipLaunchKernelGGL((k_mac_dyadic32<F, G, M>), grid, dim3(kD32Threads), 0, s, a); break;
+		D32_CASE(7, true, true, true)
+		D32_CASE(5, false, true, true)
+		D32_CASE(3, true, true, false)
+		D32_CASE(2, true, false, false)
+		D32_CASE(1, false, true, false)
+		default: hipLaunchKernelGGL((k_mac_dyadic32<false, false, false>), grid, dim3(kD32Threads), 0, s, a); break;
+#undef D32_CASE
+	}
+	return hipGetLastError();
 }
 
 // Transformed coefficients of every (output group q, input block b) of the

@@ -123,6 +123,12 @@ hipError_t launch_mac_dyadic(const DyadicArgs& a, uint64_t groups, int wpt, int 
 hipError_t launch_dyadic16_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uint64_t xbase,
                                   uint64_t ybase, hipStream_t s);
 hipError_t launch_mac_dyadic16(const DyadicArgs& a, uint64_t groups, int wpt, hipStream_t s);
+// The 32 x 32 form: blocks and output groups of 32 points, tiles of 512 words
+// (one word per thread, 512 threads), gp with 32 words per (q, b), lost masks
+// one u32 per block of 32 (with gp only).
+hipError_t launch_dyadic32_coeffs(uint64_t* out, uint64_t nblocks, uint64_t Q, uint64_t xbase,
+                                  uint64_t ybase, hipStream_t s);
+hipError_t launch_mac_dyadic32(const DyadicArgs& a, uint64_t groups, hipStream_t s);
 hipError_t launch_cauchy_points(uint64_t* out, const uint64_t* xs, uint64_t nx,
                                 const uint64_t* ys, uint64_t ny, hipStream_t s);
 hipError_t launch_mac_tab(const MacArgs& a, uint64_t groups, int rt, int wpt, int ib,

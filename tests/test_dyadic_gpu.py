@@ -62,8 +62,8 @@ def test_dyadic_equals_oracle(gpu, oracle, n, w, r, fi, fr, acc):
     try:
         for cache in (1, 0):  # coefficients from the cache / computed in the kernel
             gpu.check(lib().parhip_set_coeff_cache(cache))
-            for m in (gpu.METHOD_DYADIC, gpu.METHOD_DYADIC8, gpu.METHOD_DYADIC16, gpu.METHOD_TABLE,
-                      gpu.METHOD_AUTO):
+            for m in (gpu.METHOD_DYADIC, gpu.METHOD_DYADIC8, gpu.METHOD_DYADIC16, gpu.METHOD_DYADIC32,
+                      gpu.METHOD_TABLE, gpu.METHOD_AUTO):
                 _method(gpu, m)
                 out = d_out.clone()
                 device.recovery(d_in, r, fi, fr, out=out, accumulate=acc)
@@ -123,6 +123,34 @@ def test_dyadic16_launch_shapes(gpu, oracle, wpt, groups, cache):
         assert np.array_equal(out.cpu().numpy().view(np.uint64), expect), (n, w, r, fi, fr)
 
 
+@pytest.mark.parametrize("groups", [1, 3, 40])
+@pytest.mark.parametrize("cache", [1, 0])
+def test_dyadic32_launch_shapes(gpu, oracle, groups, cache):
+    """The 32 x 32 form: split-K rows through partials (3) and atomics (40),
+    coefficients from the cache or the kernel; full tiles (2560 = 5 x 512
+    words) and ragged ones (1000 words); one and two output groups, points off
+    the 32-point grid on both sides and past 2^40."""
+    import torch
+    from parparpar_amd import device, lib
+    for n, w, r, fi, fr in ((331, 2560, 20, 5, 331), (97, 1000, 33, 13, 2 ** 40 + 7),
+                            (130, 1536, 64, 31, 1)):
+        rng = np.random.default_rng(groups * 1000 + n + cache)
+        inp = rand_words(rng, n, w)
+        expect = oracle.recovery(inp, r, fi, fr)
+        d_in = torch.from_numpy(inp.view(np.int64)).cuda()
+        _method(gpu, gpu.METHOD_DYADIC32)
+        try:
+            gpu.check(lib().parhip_set_coeff_cache(cache))
+            gpu.check(lib().parhip_tune_override(0, 0, 0, groups))
+            out = device.recovery(d_in, r, fi, fr)
+            torch.cuda.synchronize()
+        finally:
+            gpu.check(lib().parhip_tune_override(0, 0, 0, 0))
+            gpu.check(lib().parhip_set_coeff_cache(1))
+            _method(gpu, gpu.METHOD_AUTO)
+        assert np.array_equal(out.cpu().numpy().view(np.uint64), expect), (n, w, r, fi, fr)
+
+
 def test_dyadic_reference_collision_shape(gpu, oracle):
     """The reference's overlapping-range case (test/par3-fenger-padded-engine.js:110-112):
     N=1000, R=512, 128-byte blocks, inputs at 0x10000, recovery at 66300."""
@@ -150,6 +178,34 @@ def test_dyadic_streamed_host_entry(gpu, oracle):
     finally:
         binding.set_stream_chunk(0)
     assert np.array_equal(out, oracle.recovery(inp, r, 3, n + 5))
+
+
+def test_full_size_dyadic32_equals_dyadic16(gpu, oracle):
+    """1 GiB as 1024 x 1 MiB at R = 32: the whole outputs of the 32 x 32 and
+    16 x 16 XOR-block kernels agree, and sampled word columns equal the
+    oracle."""
+    import torch
+    from parparpar_amd import device
+    n, w, r = 1024, 131072, 32
+    print("full-size dyadic32 vs dyadic16 N=%d W=%d R=%d" % (n, w, r), flush=True)
+    inp = _torch_words(n + w + r, n, w)
+    try:
+        _method(gpu, gpu.METHOD_DYADIC16)
+        d16 = device.recovery(inp, r, 0, n)
+        _method(gpu, gpu.METHOD_DYADIC32)
+        d32 = device.recovery(inp, r, 0, n)
+    finally:
+        _method(gpu, gpu.METHOD_AUTO)
+    torch.cuda.synchronize()
+    assert torch.equal(d16, d32)
+    rng = np.random.default_rng(w + r)
+    cols = np.unique(np.concatenate([[0, w - 1], rng.integers(0, w, size=6)])).astype(np.int64)
+    tc = torch.as_tensor(cols, device="cuda")
+    sample = np.ascontiguousarray(inp[:, tc].cpu().numpy().view(np.uint64))
+    ref = oracle.recovery_with_coeff(sample, oracle.cauchy(n, r, 0, n))
+    assert np.array_equal(d32[:, tc].cpu().numpy().view(np.uint64), ref)
+    del inp, d16, d32
+    torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("n,w,r", [(1024, 131072, 8), (262144, 512, 8)])
