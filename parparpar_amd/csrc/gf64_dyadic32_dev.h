@@ -22,8 +22,8 @@
 // blocks and are transformed back once before the store.
 //
 // The inputs of the next block are loaded into an input's registers as soon
-// as its last table step has taken its nibble offsets, so the loads of a block
-// overlap the rest of its lookups.
+// as its first table step has taken its nibble offsets, so the loads of a
+// block overlap the rest of its lookups.
 //
 // Points that are not multiples of 32 get virtual points in front (zero
 // inputs, outputs not stored), as in the other forms.  DyadicArgs: nblocks,
@@ -62,7 +62,7 @@ constexpr D32Steps d32_make_steps() {
 	                        {0, 1, 0, 0}, {4, 0, 0, 0}, {0, 0, 0, 0}, {-1, 0, 0, 0}};
 	D32Steps s{};
 	int i = 0;
-	for (int T = 0; T < 32; T++)
+	for (int T = 31; T >= 0; T--)
 		for (int h = 0; h < 2; h++)
 			for (int k = 0; k < nt[T & 7]; k++)
 				for (int sg = 0; sg < 4; sg++)
@@ -214,7 +214,7 @@ __device__ __forceinline__ void d32_nib16x8(uint32_t x, uint32_t (&ad)[8]) {
 }
 
 // State of one word's pass over a block: the nibble offsets of the current
-// input's first and second half (the second carrying the half's LDS base in
+// input's first and second half (the second carrying its half's LDS base in
 // its upper bytes), and two steps of reads.
 struct D32Pass {
 	uint32_t ad[2][8];
@@ -226,8 +226,12 @@ struct D32Pass {
 template <int I>
 __device__ __forceinline__ void d32_issue(const uint4* tab, const uint2 (&x)[32], D32Pass& P) {
 	constexpr int T = kD32.T[I], h = kD32.h[I], sg = kD32.sg[I], tb = kD32.tb[I];
-	constexpr bool first = I == 0 || kD32.T[I > 0 ? I - 1 : 0] != T || kD32.h[I > 0 ? I - 1 : 0] != h;
-	if constexpr (first) d32_nib16x8(h ? x[T].y : x[T].x, P.ad[h]);
+	// the first step of input T takes the offsets of both halves
+	constexpr bool first = I == 0 || kD32.T[I > 0 ? I - 1 : 0] != T;
+	if constexpr (first) {
+		d32_nib16x8(x[T].x, P.ad[0]);
+		d32_nib16x8(x[T].y, P.ad[1]);
+	}
 	const char* base = reinterpret_cast<const char*>(tab) + (sg * kDyTables) * 8 * 256;
 	if constexpr (tb >= 0) {
 #pragma unroll
@@ -260,8 +264,10 @@ __device__ __forceinline__ void d32_fold(const D32Pass& P, uint2 (&acc)[32]) {
 }
 
 // Steps I.. of a word: step I + 1's reads issued before step I is folded.
-// Once the step that takes input T's second-half offsets has been issued, x[T]
-// is free and receives input T of the next block (b_next).
+// Once the first step of input T has taken its offsets, x[T] is free and
+// receives input T of the next block (b_next); the inputs run from T = 31
+// (two steps) to T = 0 (32 steps), so the last load of a pass is issued 32
+// steps before its end.
 template <int I, bool FAST>
 __device__ __forceinline__ void d32_steps(const DyadicArgs& a, const uint4* tab, uint2 (&x)[32], D32Pass& P,
                                           uint2 (&acc)[32], uint64_t b_next, uint64_t b_end, uint64_t w,
@@ -269,7 +275,7 @@ __device__ __forceinline__ void d32_steps(const DyadicArgs& a, const uint4* tab,
 	if constexpr (I + 1 < kD32Steps) {
 		d32_issue<I + 1>(tab, x, P);
 		constexpr int T1 = kD32.T[I + 1];
-		constexpr bool frees = kD32.h[I + 1] == 1 && kD32.h[I] == 0;  // first step of (T1, 1)
+		constexpr bool frees = kD32.T[I] != T1;  // first step of input T1
 		if constexpr (frees) x[T1] = d32_load<FAST>(a, b_next, b_end, w, lost_next, T1);
 	}
 	__builtin_amdgcn_sched_barrier(0);
@@ -321,6 +327,7 @@ __device__ __forceinline__ void d32_tile(const DyadicArgs& a, uint4* tab, uint64
 			P.ad[1][kk] = kD32HalfBytes;
 		}
 		d32_issue<0>(tab, x, P);
+		x[kD32.T[0]] = d32_load<FAST>(a, bb + 1, b_end, w, lost_next, kD32.T[0]);
 		d32_steps<0, FAST>(a, tab, x, P, acc, bb + 1, b_end, w, lost_next);
 	}
 
