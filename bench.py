@@ -709,11 +709,13 @@ def measure_batch(args):
     out = {
         "workload": "batched uneven PAR3 create: %d jobs, %.2f GiB total, size LogUniform[4 MiB, 512 MiB], "
                     "slice {4,8,32,1024} KiB, R U{4..64}" % (len(jobs), gib),
-        "method": "persistent kernels, one per tile family (tables, 8x8 and 16x16 XOR blocks), over "
+        "method": "persistent kernels, one per tile family (tables, 8x8, 16x16 and 32x32 XOR blocks), over "
                   "device tile queues, in one hipGraph; per shape: XOR-block tiles (block size by "
                   "table reads, tile width autotuned), or the GF(2) bit-matrix on MFMA where it wins",
         "plain": "one parhip_d_gf64_recovery call per job (the same XOR-block kernels where R >= 4)",
         "jobs_16x16_blocks": sum(b.job_config(k)[0] == 16 and b.job_method(k) == P.METHOD_DYADIC
+                                 for k in range(len(jobs))),
+        "jobs_32x32_blocks": sum(b.job_config(k)[0] == 32 and b.job_method(k) == P.METHOD_DYADIC
                                  for k in range(len(jobs))),
         "steps": steps,
         "plain_GiBps": round(gib / res["plain"], 3),

@@ -42,11 +42,15 @@ struct CfgInfo {
 static const CfgInfo kCfg[BATCH_CFG_COUNT] = {
 	{8, 4, 4}, {8, 2, 4}, {4, 2, 4} /* MFMA */, {16, 2, 2}, {4, 4, 4},
 	{8, 4, 1} /* dyadic, one 8-output group per tile */, {8, 2, 1} /* dyadic */,
-	{16, 1, 1} /* dyadic 16 x 16, one 16-output group per tile */};
+	{16, 1, 1} /* dyadic 16 x 16, one 16-output group per tile */,
+	{32, 1, 1} /* dyadic 32 x 32: 512 threads, one 32-output group per tile */};
 
 static bool is_dyadic(int cfg) {
-	return cfg == BATCH_CFG_DY4 || cfg == BATCH_CFG_DY2 || cfg == BATCH_CFG_DY16;
+	return cfg == BATCH_CFG_DY4 || cfg == BATCH_CFG_DY2 || cfg == BATCH_CFG_DY16 || cfg == BATCH_CFG_DY32;
 }
+
+// words per tile of a variant (a 32 x 32 tile: 512 threads of one word)
+static uint64_t tile_words(int cfg) { return cfg == BATCH_CFG_DY32 ? 512 : 256ull * kCfg[cfg].wpt; }
 
 // XOR-block tiles run the straight-line (full, 16-byte aligned tile) form of
 // the kernel: slices a multiple of 512 words, inputs 16-byte aligned at an
@@ -57,16 +61,17 @@ static bool dyadic_ok(const BatchJob& jb) {
 	       (jb.in_stride & 1) == 0;
 }
 static int dyadic_default(const BatchJob& jb) {
+	if (jb.bits == 5) return BATCH_CFG_DY32;
 	if (jb.bits == 4) return BATCH_CFG_DY16;
 	return jb.W % 1024 == 0 ? BATCH_CFG_DY4 : BATCH_CFG_DY2;
 }
 
 // A dyadic job's block size, fixed when the batch is made (its coefficient
-// layout depends on it): 16 x 16 blocks where they need fewer table reads
-// (the plain create's rule) and the slices fill 256-word tiles.
+// layout depends on it): 16 x 16 or 32 x 32 blocks where they need fewer table
+// reads (the plain create's rule; XOR-block jobs fill 512-word tiles).
 static void set_job_geometry(BatchJob& jb) {
-	jb.bits = (dyadic_ok(jb) && jb.W % 256 == 0 && dyadic16_better(jb.R, jb.fr)) ? 4 : 3;
-	const uint64_t m = jb.bits == 4 ? 15 : 7;
+	jb.bits = dyadic_ok(jb) ? (uint64_t)dyadic_block_bits(jb.R, jb.fr, 5) : 3;
+	const uint64_t m = (1ull << jb.bits) - 1;
 	jb.pre_in = jb.fi & m;
 	jb.pre_out = jb.fr & m;
 	jb.nblocks = (jb.pre_in + jb.N + m) / (m + 1);
@@ -76,6 +81,10 @@ static void set_job_geometry(BatchJob& jb) {
 // The job's coefficient units (8 words each) from unit `off`: its first unit
 // (even for 16 x 16 blocks) and the next free unit.
 static uint64_t gp_units(const BatchJob& jb, uint64_t off, uint64_t* first) {
+	if (jb.bits == 5) {
+		*first = (off + 3) & ~3ull;
+		return *first + 4 * jb.Q * jb.nblocks;
+	}
 	if (jb.bits == 4) {
 		*first = (off + 1) & ~1ull;
 		return *first + 2 * jb.Q * jb.nblocks;
@@ -124,13 +133,15 @@ static const uint64_t kTileWork = 1ull << 22;  // word x output MACs per tile
 // XOR-block tiles: a block is 8 inputs x 8 outputs per word; a tile is
 // about twice a table tile's MACs (the method takes ~half the time per MAC).
 // A 16 x 16 block has 3x the products of an 8 x 8 one: as many products per
-// tile.
+// tile.  A 32 x 32 block has 9x: as many products per thread as an 8 x 8
+// tile of 512 words, so twice per work-group (one per CU instead of two).
 static uint32_t plan_job_dyadic(uint32_t jid, const BatchJob& jb, int cfg,
                                 std::vector<BatchTile>& out, bool* atomic) {
-	const uint64_t tw = 256ull * kCfg[cfg].wpt;
+	const uint64_t tw = tile_words(cfg);
 	const uint64_t wt = jb.W / tw;
 	uint64_t chunk = std::max<uint64_t>(2, (2 * kTileWork) / (tw * 64));
 	if (cfg == BATCH_CFG_DY16) chunk = std::max<uint64_t>(2, (2 * kTileWork) / (tw * 64) * 27 / 81);
+	if (cfg == BATCH_CFG_DY32) chunk = std::max<uint64_t>(2, (4 * kTileWork) / (tw * 64) * 27 / 243);
 	if (chunk >= jb.nblocks) chunk = jb.nblocks;
 	const uint64_t nch = (jb.nblocks + chunk - 1) / chunk;
 	*atomic = nch > 1;
@@ -221,7 +232,7 @@ static int run_single(const BatchJob& jb, int cfg, BatchJob* djob, BatchTile* dt
 	hipEventCreate(&a);
 	hipEventCreate(&b);
 	float best = 1e30f;
-	uint32_t fam[kBatchFamilies] = {0, 0, 0};
+	uint32_t fam[kBatchFamilies] = {};
 	fam[batch_family(cfg)] = (uint32_t)tiles.size();
 	for (int rep = 0; rep < 3; rep++) {
 		hipEventRecord(a, s);
@@ -249,7 +260,7 @@ static int run_single(const BatchJob& jb, int cfg, BatchJob* djob, BatchTile* dt
 // (dyadic jobs).
 static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_out) {
 	const bool dy = dyadic_ok(jb);
-	TuneKey key{jb.W, jb.R, n_bucket(jb.N), dy ? (jb.bits == 4 ? 2 : 1) : 0};
+	TuneKey key{jb.W, jb.R, n_bucket(jb.N), dy ? (int)jb.bits - 2 : 0};
 	{
 		std::lock_guard<std::mutex> lk(g_tune_mu);
 		auto it = g_tune.find(key);
@@ -262,9 +273,11 @@ static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_
 	if (dy) {
 		// the XOR-block method does ~half the LDS work of the direct tables
 		// per MAC at every R measured (4-64); only its tile width is tuned
-		// (16 x 16 blocks: one width)
+		// (16 x 16 and 32 x 32 blocks: one width)
 		cand.clear();
-		if (jb.bits == 4) {
+		if (jb.bits == 5) {
+			cand.push_back(BATCH_CFG_DY32);
+		} else if (jb.bits == 4) {
 			cand.push_back(BATCH_CFG_DY16);
 		} else {
 			if (jb.W % 1024 == 0) cand.push_back(BATCH_CFG_DY4);
@@ -321,7 +334,8 @@ static int autotune_job(const BatchJob& jb, int blocks, hipStream_t s, int* cfg_
 		DyadicPlan dp{};
 		DevBuf dws;
 		if (dy) {
-			dp = plan_dyadic(jb.N, jb.R, jb.W, jb.fi, jb.fr, current_device(), SIZE_MAX, jb.bits == 4);
+			dp = plan_dyadic(jb.N, jb.R, jb.W, jb.fi, jb.fr, current_device(), SIZE_MAX, true, (int)jb.bits,
+			                 true);
 			if ((rc = dws.alloc(dp.part_bytes, s))) return rc;
 		}
 		float ms_tab = 1e30f, ms_mfma = 1e30f;
@@ -376,7 +390,7 @@ struct parhip_batch {
 	BatchJob* djobs = nullptr;
 	BatchTile* dtiles = nullptr;
 	uint32_t ntiles = 0;
-	uint32_t fam_tiles[kBatchFamilies] = {0, 0, 0};  // tiles of each family, grouped in dtiles
+	uint32_t fam_tiles[kBatchFamilies] = {};  // tiles of each family, grouped in dtiles
 	uint32_t* dcounter = nullptr;                      // one queue counter per family
 	uint32_t* dzero = nullptr;
 	uint32_t nzero = 0;
@@ -673,7 +687,8 @@ void parhip_batch_destroy(parhip_batch* b) { batch_free(b); }
 // The entry of a shape: the XOR-block one where the shape qualifies for it
 // (aligned slices of a multiple of 512 words, R >= 4), else the other.
 static std::map<TuneKey, TuneVal>::iterator tune_find(size_t N, size_t R, size_t W) {
-	auto it = g_tune.find(TuneKey{W, R, n_bucket(N), 2});
+	auto it = g_tune.find(TuneKey{W, R, n_bucket(N), 3});
+	if (it == g_tune.end()) it = g_tune.find(TuneKey{W, R, n_bucket(N), 2});
 	if (it == g_tune.end()) it = g_tune.find(TuneKey{W, R, n_bucket(N), 1});
 	if (it == g_tune.end()) it = g_tune.find(TuneKey{W, R, n_bucket(N), 0});
 	return it;
@@ -728,8 +743,8 @@ int parhip_autotune_load(const char* path) {
 		double ns;
 		// an XOR-block variant only under a key that qualifies for it
 		if (sscanf(line, "%llu %llu %llu %d %d %lg", &W, &R, &nb, &dy, &cfg, &ns) == 6 && cfg >= 0 &&
-		    cfg < BATCH_CFG_COUNT && kCfg[cfg].rt && dy >= 0 && dy <= 2 &&
-		    (is_dyadic(cfg) ? (cfg == BATCH_CFG_DY16) == (dy == 2) && dy : true))
+		    cfg < BATCH_CFG_COUNT && kCfg[cfg].rt && dy >= 0 && dy <= 3 &&
+		    (is_dyadic(cfg) ? dy == (cfg == BATCH_CFG_DY32 ? 3 : cfg == BATCH_CFG_DY16 ? 2 : 1) : true))
 			g_tune[TuneKey{W, R, nb, dy}] = {cfg, ns};
 	}
 	fclose(f);

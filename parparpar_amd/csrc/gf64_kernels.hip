@@ -18,6 +18,7 @@
 #include <stdint.h>
 #include "gf64_dev.h"
 #include "gf64_dyadic16_dev.h"
+#include "gf64_dyadic32_dev.h"
 #include "gf64_dyadic_dev.h"
 #include "kernels.h"
 
@@ -372,8 +373,47 @@ __device__ __forceinline__ void run_batch_dy16(const BatchJob& jb, const BatchTi
 	                        tl.mode == OUT_ATOMIC ? DY_OUT_ATOMIC : DY_OUT_DIRECT);
 }
 
+// A 32 x 32 XOR-block tile (gf64_dyadic32_dev.h): 512 words, output group r0
+// of 32 points, blocks [i0, i1) of 32 points; 112 KiB of tables in `tab`.
+__device__ __forceinline__ void run_batch_dy32(const BatchJob& jb, const BatchTile& tl, uint4* tab) {
+	DyadicArgs a{};
+	a.in_base = jb.in;
+	a.in_stride = jb.in_stride;
+	a.out_base = jb.out;
+	a.out_stride = jb.out_stride;
+	a.gp = jb.gp;
+	a.gp_ld = jb.nblocks;
+	a.N = jb.N;
+	a.R = jb.R;
+	a.W = jb.W;
+	a.pre_in = jb.pre_in;
+	a.pre_out = jb.pre_out;
+	a.nblocks = jb.nblocks;
+	a.accumulate = 0;
+	d32_tile<true, true, false>(a, tab, nullptr, tl.r0, tl.w0, tl.i0, tl.i1, 0,
+	                            tl.mode == OUT_ATOMIC ? DY_OUT_ATOMIC : DY_OUT_DIRECT);
+}
+
+// The 32 x 32 family: one work-group of 512 threads per CU, the same queue
+// protocol as k_batch_persistent.
+__global__ __launch_bounds__(kD32Threads, 1) void k_batch_persistent32(const BatchJob* __restrict__ jobs,
+                                                                   const BatchTile* __restrict__ tiles,
+                                                                   uint32_t ntiles, uint32_t* counter) {
+	__shared__ uint4 tab[4 * kDyTables * 256];  // four sets of seven pair tables
+	__shared__ uint32_t t_next;
+	for (;;) {
+		if (threadIdx.x == 0) t_next = atomicAdd(counter, 1u);
+		__syncthreads();
+		const uint32_t t = t_next;
+		__syncthreads();
+		if (t >= ntiles) break;
+		const BatchTile tl = tiles[t];
+		run_batch_dy32(jobs[tl.job], tl, tab);
+	}
+}
+
 // One family of tiles (batch_family): FAM 0 the table variants, 1 the 8 x 8
-// XOR-block tiles, 2 the 16 x 16 ones.
+// XOR-block tiles, 2 the 16 x 16 ones (the 32 x 32 ones: k_batch_persistent32).
 template <int FAM>
 __global__ __launch_bounds__(256, 2) void k_batch_persistent(const BatchJob* __restrict__ jobs,
                                                           const BatchTile* __restrict__ tiles,
@@ -431,9 +471,10 @@ __global__ void k_cauchy_batch(const BatchJob* __restrict__ jobs, uint32_t njobs
 
 // XOR-block coefficients of every dyadic job of a batch: 8 threads per
 // 8-word unit, job found by binary search over jobs[j].gp_off; the same values
-// as k_dyadic_coeffs (bits 3: a unit is one (q, block) pair) and
+// as k_dyadic_coeffs (bits 3: a unit is one (q, block) pair),
 // k_dyadic16_coeffs (bits 4: units 2p, 2p+1 are pair p, from an even unit, so
-// a pair's 16 threads are 16-aligned in the work-group).
+// a pair's 16 threads are 16-aligned in the work-group) and k_dyadic32_coeffs
+// (bits 5: units 4p..4p+3 are pair p, from a multiple of 4: 32-aligned).
 __global__ __launch_bounds__(256) void k_dyadic_coeffs_batch(const BatchJob* __restrict__ jobs,
                                                              uint32_t njobs, uint64_t total) {
 	__shared__ uint64_t gs[256];
@@ -443,7 +484,7 @@ __global__ __launch_bounds__(256) void k_dyadic_coeffs_batch(const BatchJob* __r
 		const uint64_t idx = base_unit + (threadIdx.x >> 3);
 		uint64_t c = 0;
 		uint32_t lo = 0;
-		bool b16 = false;
+		int bits = 3;
 		if (idx < total) {
 			uint32_t hi = njobs - 1;
 			while (lo < hi) {
@@ -452,9 +493,12 @@ __global__ __launch_bounds__(256) void k_dyadic_coeffs_batch(const BatchJob* __r
 			}
 			const BatchJob& jb = jobs[lo];
 			const uint64_t e = idx - jb.gp_off;
-			b16 = jb.bits == 4;
+			bits = (int)jb.bits;
 			uint64_t z;
-			if (b16) {
+			if (bits == 5) {
+				const uint64_t pr = e >> 2, q = pr / jb.nblocks, b = pr - q * jb.nblocks;
+				z = (((jb.fi & ~31ull) + 32 * b) ^ ((jb.fr & ~31ull) + 32 * q)) ^ (uint64_t)(8 * (e & 3) + u);
+			} else if (bits == 4) {
 				const uint64_t pr = e >> 1, q = pr / jb.nblocks, b = pr - q * jb.nblocks;
 				z = (((jb.fi & ~15ull) + 16 * b) ^ ((jb.fr & ~15ull) + 16 * q)) ^ (uint64_t)(8 * (e & 1) + u);
 			} else {
@@ -466,7 +510,11 @@ __global__ __launch_bounds__(256) void k_dyadic_coeffs_batch(const BatchJob* __r
 		gs[threadIdx.x] = c;
 		__syncthreads();
 		uint64_t acc = 0;
-		if (b16) {
+		if (bits == 5) {
+			const int uu = threadIdx.x & 31;
+			for (int t = 0; t < 32; t++)
+				if ((t & uu) == uu) acc ^= gs[(threadIdx.x & ~31) + t];
+		} else if (bits == 4) {
 			const int uu = threadIdx.x & 15;
 #pragma unroll
 			for (int t = 0; t < 16; t++)
@@ -660,9 +708,15 @@ hipError_t launch_batch(const BatchJob* jobs, uint32_t njobs, const BatchTile* t
 		for (int f = 0; f < kBatchFamilies; f++) {
 			const uint32_t n = fam_tiles[f];
 			if (!n) continue;
-			void (*k)(const BatchJob*, const BatchTile*, uint32_t, uint32_t*) =
-			    f == 0 ? k_batch_persistent<0> : (f == 1 ? k_batch_persistent<1> : k_batch_persistent<2>);
-			hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, s, jobs, tiles + off, n, counters + f);
+			if (f == 3) {
+				const int b32 = blocks > 1 ? blocks / 2 : 1;
+				hipLaunchKernelGGL(k_batch_persistent32, dim3((unsigned)b32), dim3(kD32Threads), 0, s, jobs,
+				                   tiles + off, n, counters + f);
+			} else {
+				void (*k)(const BatchJob*, const BatchTile*, uint32_t, uint32_t*) =
+				    f == 0 ? k_batch_persistent<0> : (f == 1 ? k_batch_persistent<1> : k_batch_persistent<2>);
+				hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(256), 0, s, jobs, tiles + off, n, counters + f);
+			}
 			off += n;
 		}
 	}

@@ -33,7 +33,8 @@ enum BatchCfg : uint16_t {
 	BATCH_CFG_DY4 = 5,     // XOR-block (dyadic) tiles, WPT=4 (full 1024-word tiles)
 	BATCH_CFG_DY2 = 6,     // XOR-block (dyadic) tiles, WPT=2 (full 512-word tiles)
 	BATCH_CFG_DY16 = 7,    // 16 x 16 XOR-block tiles, WPT=1 (full 256-word tiles)
-	BATCH_CFG_COUNT = 8
+	BATCH_CFG_DY32 = 8,    // 32 x 32 XOR-block tiles, 512 threads x 1 word (full 512-word tiles)
+	BATCH_CFG_COUNT = 9
 };
 struct BatchJob {
 	const uint64_t* in;
@@ -45,17 +46,19 @@ struct BatchJob {
 	uint64_t fi, fr;
 	uint64_t coeff_off;   // first global index of this job's coefficients (0 of them for dyadic jobs)
 	// dyadic jobs: Q x nblocks x 2^bits XOR-block coefficients (k_dyadic_coeffs
-	// layout; bits 4: k_dyadic16_coeffs layout); nblocks, Q, pre_in, pre_out
-	// count in runs of 2^bits points
+	// layout; bits 4: k_dyadic16_coeffs layout; bits 5: k_dyadic32_coeffs
+	// layout); nblocks, Q, pre_in, pre_out count in runs of 2^bits points
 	uint64_t* gp;
 	uint64_t nblocks, Q, pre_in, pre_out;
-	// first global 8-word unit of this job's coefficients (Q*nblocks units, or
-	// 2*Q*nblocks from an even unit for bits 4; 0 for other jobs)
+	// first global 8-word unit of this job's coefficients (Q*nblocks units,
+	// 2*Q*nblocks from an even unit for bits 4, 4*Q*nblocks from a multiple
+	// of 4 for bits 5; 0 for other jobs)
 	uint64_t gp_off;
-	uint64_t bits;        // 3: 8 x 8 blocks, 4: 16 x 16 blocks
+	uint64_t bits;        // 3: 8 x 8 blocks, 4: 16 x 16 blocks, 5: 32 x 32 blocks
 };
 // Table tiles: outputs [r0, r0+RT), inputs [i0, i1).  Dyadic tiles: output
-// group r0, input blocks [i0, i1) (of 8 or 16 points).  Both: words [w0, w0 + 256*WPT).
+// group r0, input blocks [i0, i1) (of 8, 16 or 32 points).  Words [w0, w0 + 256*WPT)
+// (32 x 32 tiles: [w0, w0 + 512)).
 struct BatchTile {
 	uint32_t job;
 	uint16_t cfg;         // BatchCfg
@@ -69,12 +72,16 @@ struct BatchTile {
 // the dyadic jobs), zeroing of the atomically accumulated jobs' outputs,
 // queue reset, persistent kernel.
 // Tiles come grouped by family (table tiles, 8 x 8 XOR-block tiles, 16 x 16
-// XOR-block tiles; fam_tiles[f] of each, in that order), and each family runs
-// as its own persistent kernel over its own queue counter (counters[f]): one
-// kernel holding every variant would size its registers for the largest and
-// spill in the others.
-constexpr int kBatchFamilies = 3;
+// XOR-block tiles, 32 x 32 XOR-block tiles; fam_tiles[f] of each, in that
+// order), and each family runs as its own persistent kernel over its own queue
+// counter (counters[f]): one kernel holding every variant would size its
+// registers for the largest and spill in the others.  `blocks` is the
+// persistent grid of the 256-thread families (two work-groups per CU); the
+// 32 x 32 family runs half as many work-groups of 512 threads (one per CU,
+// 112 KiB of LDS each).
+constexpr int kBatchFamilies = 4;
 inline int batch_family(int cfg) {
+	if (cfg == BATCH_CFG_DY32) return 3;
 	return cfg == BATCH_CFG_DY16 ? 2 : ((cfg == BATCH_CFG_DY4 || cfg == BATCH_CFG_DY2) ? 1 : 0);
 }
 hipError_t launch_batch(const BatchJob* jobs, uint32_t njobs, const BatchTile* tiles,

@@ -21,7 +21,7 @@ def _jobs(rng, shapes, torch):
 SHAPES = [
     (1, 1, 1, 0, 1), (3, 5, 2, 0, 3), (64, 8192, 8, 0, 64), (1000, 512, 4, 0, 1000),
     (5000, 512, 33, 7, 9000), (16, 131072, 8, 0, 16), (77, 1024, 64, 0, 77),
-    (300, 4096, 17, 2, 302), (2, 33, 5, 100, 0),
+    (300, 4096, 17, 2, 302), (2, 33, 5, 100, 0), (2000, 512, 32, 5, 4096),
 ]
 
 
