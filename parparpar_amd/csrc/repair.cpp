@@ -93,7 +93,7 @@ static size_t repair_ws_bytes(uint64_t N, uint64_t W, uint64_t k, const DyadicPl
 	size_t b = 0;
 	if (dp) {
 		b += align256(dp->nblocks * 4);             // lost-input masks
-		b += align256(dp->nblocks * dp->Q * 8 * (dp->bits == 4 ? 16 : 8));  // XOR-block coefficients
+		b += align256(dp->nblocks * dp->Q * 8 * (1ull << dp->bits));  // XOR-block coefficients
 		b += align256(dp->part_bytes);              // split-K partials of the RHS
 		b += align256(k * 8) * 2;                   // xs_missing, ys
 	} else {
@@ -116,7 +116,7 @@ static size_t repair_bytes(uint64_t N, uint64_t W, uint64_t k, uint64_t fi, uint
 	const int dev = current_device();
 	const size_t solve = plan_workspace_bytes(plan_contraction(k, k, W, dev), k, W);
 	if (repair_dyadic(nullptr, k, auto_method)) {
-		const DyadicPlan dp = plan_dyadic(N, k, W, fi, fr, dev, SIZE_MAX, true, force_bits);
+		const DyadicPlan dp = plan_dyadic(N, k, W, fi, fr, dev, SIZE_MAX, true, force_bits, true);
 		return repair_ws_bytes(N, W, k, &dp) + solve;
 	}
 	return repair_ws_bytes(N, W, k, nullptr) +
@@ -187,8 +187,8 @@ static int repair_enqueue(const RepairArchive& ar, const uint64_t* missing, uint
 	}
 	const uint64_t yfirst = ar.fr + (rows ? rows[0] : 0);
 	DyadicPlan dp{};
-	if (dy) dp = plan_dyadic(N, k, W, ar.fi, yfirst, current_device(), SIZE_MAX, true);
-	const uint64_t bm = dp.bits == 4 ? 15 : 7;  // points per XOR block - 1
+	if (dy) dp = plan_dyadic(N, k, W, ar.fi, yfirst, current_device(), SIZE_MAX, true, 0, true);
+	const uint64_t bm = (1ull << dp.bits) - 1;  // points per XOR block - 1 (blocks of 8, 16 or 32)
 	// workspace carve-up
 	plan.ws_bytes = repair_ws_bytes(N, W, k, dy ? &dp : nullptr);
 	int rc = dev_alloc(&plan.ws, plan.ws_bytes, s);
@@ -239,8 +239,9 @@ static int repair_enqueue(const RepairArchive& ar, const uint64_t* missing, uint
 	if (dy) {
 		// RHS = the rows used, XORed with the contraction of every input but
 		// the lost ones (masked to zero) at the rows' points
-		e = dp.bits == 4 ? launch_dyadic16_coeffs(d_gp, dp.nblocks, dp.Q, ar.fi & ~bm, yfirst & ~bm, s)
-		                 : launch_dyadic_coeffs(d_gp, dp.nblocks, dp.Q, ar.fi & ~bm, yfirst & ~bm, s);
+		e = dp.bits == 5   ? launch_dyadic32_coeffs(d_gp, dp.nblocks, dp.Q, ar.fi & ~bm, yfirst & ~bm, s)
+		    : dp.bits == 4 ? launch_dyadic16_coeffs(d_gp, dp.nblocks, dp.Q, ar.fi & ~bm, yfirst & ~bm, s)
+		                   : launch_dyadic_coeffs(d_gp, dp.nblocks, dp.Q, ar.fi & ~bm, yfirst & ~bm, s);
 		if (e == hipSuccess)
 			e = hipMemcpy2DAsync(d_rhs, W * 8, ar.rec + (yfirst - ar.fr) * ar.rec_stride,
 			                     ar.rec_stride * 8, W * 8, k, hipMemcpyDeviceToDevice, s);
@@ -540,11 +541,12 @@ int parhip_d_gf64_repair(const uint64_t* d_data, size_t numInputs, size_t dataSt
 }
 
 size_t parhip_gf64_repair_workspace_bytes(size_t numInputs, size_t blockSize64, size_t k) {
-	// points in the worst alignment of either XOR-block size (7 or 15
+	// points in the worst alignment of each XOR-block size (7, 15 or 31
 	// virtual slots before the first input and the first recovery point): an
 	// upper bound for any archive
-	return std::max(repair_bytes(numInputs, blockSize64, k, 7, 7, false, 3),
-	                repair_bytes(numInputs, blockSize64, k, 15, 15, false, 4));
+	return std::max({repair_bytes(numInputs, blockSize64, k, 7, 7, false, 3),
+	                 repair_bytes(numInputs, blockSize64, k, 15, 15, false, 4),
+	                 repair_bytes(numInputs, blockSize64, k, 31, 31, false, 5)});
 }
 
 parhip_repair_server* parhip_repair_server_create(size_t hbmBudgetBytes, unsigned maxInflight,

@@ -227,8 +227,8 @@ def test_server_colliding_points(gpu):
 
 
 # The two RHS routes: the create's XOR-block contraction over the whole archive
-# with the lost inputs masked to zero (consecutive rows, k >= 4; blocks of 8
-# or 16 points) and the explicit-coefficient tables (forced with the table
+# with the lost inputs masked to zero (consecutive rows, k >= 4; blocks of 8,
+# 16 or 32 points) and the explicit-coefficient tables (forced with the table
 # method).  Points start
 # off a multiple of 8, so virtual slots precede the first input and the first
 # recovery point; lost inputs fall in one block of 8, across blocks and at the
@@ -244,6 +244,10 @@ ROUTE_CASES = [
     # of 16 and across blocks, k = 16 and 33
     (500, 1024, 48, 13, 13 + 500 + 3, list(range(2, 18)), list(range(5, 21))),
     (333, 520, 40, 29, 29 + 333, [0, 1, 2] + list(range(100, 130)), None),
+    # 32 x 32 blocks: 21 virtual slots in front, lost inputs inside one block
+    # of 32 and across blocks, k = 24 (rows from 3) and 40
+    (700, 1024, 64, 21, 21 + 700 + 5, list(range(10, 34)), list(range(3, 27))),
+    (400, 512, 50, 45, 45 + 400, [0, 1] + list(range(60, 98)), None),
 ]
 
 
@@ -264,7 +268,7 @@ def test_repair_routes_agree(gpu, case):
                                 device="cuda", generator=g)
     outs = []
     try:
-        for m in (P.METHOD_AUTO, P.METHOD_DYADIC8, P.METHOD_DYADIC16, P.METHOD_TABLE):
+        for m in (P.METHOD_AUTO, P.METHOD_DYADIC8, P.METHOD_DYADIC16, P.METHOD_DYADIC32, P.METHOD_TABLE):
             P.set_method(m)
             outs.append(repair.repair(damaged, rec, missing, first_input=fi, first_recovery=fr, rows=rows))
     finally:
