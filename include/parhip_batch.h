@@ -52,13 +52,14 @@ int parhip_batch_run(parhip_batch *b, void *stream);
 int parhip_batch_info(const parhip_batch *b, uint64_t *ntiles, uint64_t *natomic_jobs,
                       uint64_t *coeff_bytes);
 /* Launch variant chosen for job `job` (its index in the array given to
- * parhip_batch_create): outputs per work-group, words per thread, inputs per
- * LDS batch (MFMA: outputs per work-group, 32-word column blocks per wave,
- * inputs per stage).  PARHIP_ERR_ARG for an empty job (no inputs, outputs or
+ * parhip_batch_create): table tiles: outputs per work-group, words per thread,
+ * inputs per LDS batch; XOR-block tiles: points per block (8, 16 or 32), words
+ * per thread, 1; MFMA: outputs per work-group, 32-word column blocks per wave,
+ * inputs per stage.  PARHIP_ERR_ARG for an empty job (no inputs, outputs or
  * words). */
 int parhip_batch_job_config(const parhip_batch *b, size_t job, int *rt, int *wpt, int *ib);
-/* Strategy of job `job`: PARHIP_METHOD_TABLE or PARHIP_METHOD_MFMA (include/parhip.h);
- * PARHIP_ERR_ARG for an empty job. */
+/* Strategy of job `job`: PARHIP_METHOD_DYADIC (XOR-block tiles), PARHIP_METHOD_TABLE
+ * or PARHIP_METHOD_MFMA (include/parhip.h); PARHIP_ERR_ARG for an empty job. */
 int parhip_batch_job_method(const parhip_batch *b, size_t job);
 void parhip_batch_destroy(parhip_batch *b);
 
