@@ -81,7 +81,7 @@ __device__ __forceinline__ void gf16_tab_load(const Gf16Args& a, uint64_t ib0, u
 }
 
 template <int NG, int IB>
-__global__ __launch_bounds__(256) void k_gf16_tab(Gf16Args a) {
+__global__ __launch_bounds__(256, 2) void k_gf16_tab(Gf16Args a) {
 	static_assert(IB * 4 * NG <= 256, "one basis pair per thread");
 	constexpr int TILE = 256 * 4;  // words per work-group
 	__shared__ uint4 tab[IB][NG][4][16];
@@ -127,25 +127,45 @@ __global__ __launch_bounds__(256) void k_gf16_tab(Gf16Args a) {
 		}
 		__syncthreads();
 		// phase 2: tab[ib][q][k][n] = XOR of the basis rows j = 4k + b for the set
-		// bits b of n, output pairs 4q..4q+3 in the four dwords.  Lane n of each
-		// 16-lane group writes entry n of one 256-byte row: no bank conflicts.
-		for (int t = tid; t < IB * NG * 4 * 16; t += 256) {
-			const unsigned n = t & 15;
-			const int row = t >> 4;
+		// bits b of n, output pairs 4q..4q+3 in the four dwords.  Two threads per
+		// (ib, q, k) row, entries 8*hf .. 8*hf + 7 each, from the row's four basis
+		// vectors held in registers: the first entry from all four, then one XOR
+		// per entry in Gray-code order (g(s) = s ^ s >> 1 changes one bit per
+		// step).  The low three bits are offset by the row (XOR with row & 7), so
+		// the 16 lanes of a write pass (8 rows x 2 halves) write 16 different
+		// entries of their 256-byte rows: no bank conflicts.
+		for (int t = tid; t < IB * NG * 4 * 2; t += 256) {
+			const int row = t >> 1;
+			const unsigned hf = t & 1;
 			const int k = row & 3;
 			const int q = (row >> 2) % NG;
 			const int ib = (row >> 2) / NG;
+			uint4 bv[4];
+#pragma unroll
+			for (int b = 0; b < 4; b++) bv[b] = *reinterpret_cast<const uint4*>(&bas[ib][4 * k + b][4 * q]);
+			const unsigned off = (unsigned)row & 7u;
 			uint4 ent = make_uint4(0, 0, 0, 0);
 #pragma unroll
 			for (int b = 0; b < 4; b++) {
-				const uint4 v = *reinterpret_cast<const uint4*>(&bas[ib][4 * k + b][4 * q]);
-				const uint32_t m = 0u - ((n >> b) & 1u);
-				ent.x ^= v.x & m;
-				ent.y ^= v.y & m;
-				ent.z ^= v.z & m;
-				ent.w ^= v.w & m;
+				const uint32_t m = 0u - (((off | (hf << 3)) >> b) & 1u);
+				ent.x ^= bv[b].x & m;
+				ent.y ^= bv[b].y & m;
+				ent.z ^= bv[b].z & m;
+				ent.w ^= bv[b].w & m;
 			}
-			tab[ib][q][k][n] = ent;
+			uint4* dst = &tab[ib][q][k][0];
+#pragma unroll
+			for (int st = 0; st < 8; st++) {
+				if (st) {
+					// bit that g(st) flips against g(st - 1): the lowest set bit of st
+					const int fb = st & 1 ? 0 : (st & 2 ? 1 : 2);
+					ent.x ^= bv[fb].x;
+					ent.y ^= bv[fb].y;
+					ent.z ^= bv[fb].z;
+					ent.w ^= bv[fb].w;
+				}
+				dst[(hf << 3) | ((unsigned)(st ^ (st >> 1)) ^ off)] = ent;
+			}
 		}
 		__syncthreads();
 		// next batch's loads, in flight during this batch's lookups
