@@ -218,3 +218,29 @@ def test_batch_dyadic_and_table_tiles_mixed(gpu, oracle):
             elif k == 0:
                 assert torch.equal(y, first), rep
     b.close()
+
+
+def test_batch_block_sizes_follow_the_plain_rule(gpu):
+    """XOR-block jobs take 8 x 8, 16 x 16 or 32 x 32 blocks by the plain
+    create's rule (fewest table reads: R = 8 -> 8, 16 and 40 -> 16, 32 and 64
+    -> 32 with aligned points), and each job equals the plain call."""
+    import torch
+    import parparpar_amd as P
+    from parparpar_amd import device
+    from parparpar_amd.batch import Batch
+    rng = np.random.default_rng(409)
+    shapes = [(64, 512, 8, 0, 64), (64, 512, 16, 0, 64), (64, 512, 32, 0, 64),
+              (64, 512, 40, 0, 64), (64, 1024, 64, 0, 64), (100, 512, 32, 3, 4096)]
+    points = [8, 16, 32, 16, 32, 32]
+    jobs = _jobs(rng, shapes, torch)
+    b = Batch(jobs, graph=True, autotune=False)
+    for k, s in enumerate(shapes):
+        assert b.job_method(k) == P.METHOD_DYADIC, (k, s)
+        assert b.job_config(k)[0] == points[k], (k, s, b.job_config(k))
+    b.run()
+    torch.cuda.synchronize()
+    for k, (x, y, fi, fr) in enumerate(jobs):
+        ref = device.recovery(x, y.shape[0], fi, fr)
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref), (k, shapes[k])
+    b.close()
