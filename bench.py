@@ -75,6 +75,21 @@ def parse():
     return ap.parse_args()
 
 
+def gather_rate():
+    """ns per wave-level LDS read per CU (16-byte, 8-byte) for random table-entry
+    reads, measured on the device (parhip_measure_lds_gather)."""
+    import ctypes
+    import parparpar_amd as P
+    a, b = ctypes.c_double(), ctypes.c_double()
+    P.check(P.lib().parhip_measure_lds_gather(ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+def device_cus():
+    import torch
+    return torch.cuda.get_device_properties(0).multi_processor_count
+
+
 def host_cpu():
     """CPU model, logical CPUs and the CPUs this process may run on."""
     model = "unknown"
@@ -429,6 +444,18 @@ def main_par3(args):
         if ceil_ms:
             line["roofline"]["lds_ceiling_ms"] = ceil_ms
             line["roofline"]["frac_of_lds_ceiling"] = round(ceil_ms / (mac_avg_s * 1e3), 4)
+        if r <= 8 and args.method in ("auto", "dyadic", "dyadic8"):
+            # the LDS gather rate measured live on this device: a word of an 8 x 8
+            # block reads 13 x 16 16-byte and 16 8-byte table entries (one output
+            # group at R <= 8); the time those reads take at that rate
+            g128, g64 = gather_rate()
+            nblocks = (n + 7) // 8
+            reads128 = w * nblocks * 13 * 16 / 64.0
+            reads64 = w * nblocks * 16 / 64.0
+            gather_ms = (reads128 * g128 + reads64 * g64) / device_cus() / 1e6
+            line["roofline"]["lds_gather_ns_per_read"] = {"b128": round(g128, 4), "b64": round(g64, 4)}
+            line["roofline"]["lds_gather_bound_ms"] = round(gather_ms, 4)
+            line["roofline"]["frac_of_lds_gather_rate"] = round(gather_ms / (mac_avg_s * 1e3), 4)
     if args.method in ("mfma", "mfma-i8"):
         peak = 10100.0 if args.method == "mfma" else 5000.0
         tops = 2.0 * 64 * 64 * n * r * w / mac_avg_s / 1e12

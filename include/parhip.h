@@ -324,6 +324,13 @@ int parhip_prof_reset(void);
  * "cauchy". Synchronises with the recorded events. */
 int parhip_prof_get(const char *kernel, double *total_ms, uint64_t *launches);
 
+/* The device's LDS gather rate for the table kernels' access pattern, measured
+ * by a kernel that does only that (random 16-byte, or 8-byte, reads of one entry
+ * of a 16-entry 256-byte table row per lane, many waves per CU): nanoseconds per
+ * wave-level read per CU. No counterpart in the reference; for measurement
+ * (the bound of the XOR-block and table kernels). Runs on the null stream. */
+int parhip_measure_lds_gather(double *ns_b128, double *ns_b64);
+
 /* Device workspace the contraction of this shape needs (bytes). Informational. */
 size_t parhip_gf64_workspace_bytes(size_t numInputs, size_t numRecovery, size_t blockSize64);
 

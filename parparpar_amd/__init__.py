@@ -129,6 +129,7 @@ _SIGS = [
     ("parhip_prof_enable", _int, [_int]),
     ("parhip_prof_reset", _int, []),
     ("parhip_prof_get", _int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
+    ("parhip_measure_lds_gather", _int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
 ]
 
 PARHIP_OK = 0

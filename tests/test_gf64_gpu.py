@@ -507,3 +507,16 @@ def test_mul_arr_semantics_reference_cases(gpu, oracle):
     co = np.array([0xCC00CC00CC00CC00, 0xDD00DD00DD00DD00], dtype=np.uint64)
     binding.mul_arr(buf, buf, co, 4, 2)
     assert [int(v) for v in buf] == [m(int(x[i]), int(co[i % 2])) for i in range(4)]
+
+
+@pytest.mark.gpu
+def test_measure_lds_gather(gpu):
+    """The LDS gather-rate measurement returns plausible per-read times (ns per
+    wave-level read per CU), 16-byte reads no cheaper than 8-byte ones."""
+    import ctypes
+    import parparpar_amd as P
+    a, b = ctypes.c_double(), ctypes.c_double()
+    P.check(P.lib().parhip_measure_lds_gather(ctypes.byref(a), ctypes.byref(b)))
+    assert 0.2 < a.value < 50.0 and 0.1 < b.value < 50.0, (a.value, b.value)
+    assert a.value >= b.value * 0.9, (a.value, b.value)
+    assert P.lib().parhip_measure_lds_gather(None, ctypes.byref(b)) == P.PARHIP_ERR_ARG
