@@ -79,7 +79,9 @@ static void set_job_geometry(BatchJob& jb) {
 }
 
 // The job's coefficient units (8 words each) from unit `off`: its first unit
-// (even for 16 x 16 blocks) and the next free unit.
+// (even for 16 x 16 blocks, a multiple of 4 for 32 x 32 blocks, so a block's
+// coefficients are one aligned run of threads in k_dyadic_coeffs_batch) and
+// the next free unit.
 static uint64_t gp_units(const BatchJob& jb, uint64_t off, uint64_t* first) {
 	if (jb.bits == 5) {
 		*first = (off + 3) & ~3ull;
